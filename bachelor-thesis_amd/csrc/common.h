@@ -1,0 +1,49 @@
+// common.h -- internal helpers shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include "../../include/mgx.h"
+
+namespace mgx {
+
+// Thread-local error text behind mgx_last_error().
+void set_error(const char *fmt, ...);
+int fail(int status, const char *fmt, ...);
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+#define MGX_HIP(call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return ::mgx::fail(MGX_ERR_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                               __FILE__, __LINE__);                                        \
+    } while (0)
+
+#define MGX_TRY(expr)                     \
+    do {                                  \
+        int s_ = (expr);                  \
+        if (s_ != MGX_OK) return s_;      \
+    } while (0)
+
+// Device-side error word: kernels set bits, the host checks after a sync.
+enum : unsigned { DEV_ERR_CAPACITY = 1u, DEV_ERR_INPUT = 2u };
+unsigned *device_error_word();          // per-device, zero-initialised at init
+int check_device_error(hipStream_t s);  // synchronizes s, reads and clears the word
+
+// Stream-ordered scratch from the device's default memory pool.
+int scratch_alloc(void **p, size_t bytes, hipStream_t s);
+int scratch_free(void *p, hipStream_t s);
+
+int ensure_init();
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline int64_t round_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
+
+// Tunables (mgx_set_param).
+int64_t param(int which);
+
+}  // namespace mgx

@@ -1,0 +1,233 @@
+/*
+ * mgx.h -- C ABI of the MI355X exact big-integer matrix multiplier.
+ *
+ * Drop-in for the GPU path of mat-gpu (tomsmeding/bachelor-thesis).  Every
+ * entry point takes plain pointers and sizes; device pointers are HIP device
+ * memory, streams are hipStream_t passed as void* (NULL = the null stream).
+ *
+ * ENTRY TYPE.  An entry is the reference's fixed-capacity signed integer
+ * struct gpu___mpz_t {limb ptr[cap]; mp_size_t size;}
+ * (mat-gpu/gmp-es-mpz.h:22-27) in its 64-bit-limb configuration
+ * (mat-gpu/gmp-es.h:27-31): cap little-endian uint64_t limbs followed by one
+ * int64_t size word, |size| = number of limbs in use, sign of size = sign of
+ * the value, zero <=> size 0.  cap is a run-time argument here (the
+ * reference fixes it at compile time, gmp-es-mpz.h:18); one call uses one cap
+ * for all its matrices.  Outputs are always normalized (top limb non-zero);
+ * inputs may carry zero top limbs.
+ *
+ * MATRICES.  Row-major, rowstride counted in entries (mat-gpu/main.cu:26).
+ * C is K x N, A is K x M, B is M x N (mat-gpu/main.cu:242-245).
+ *
+ * INTERLEAVED ("strided") LAYOUT.  Within each block of MGX_INTERLEAVE_BLOCK
+ * consecutive entries of a row, word w of entry j is stored at word position
+ * w * MGX_INTERLEAVE_BLOCK + j (mat-gpu/main.cu:137-156,
+ * mat-gpu/mat-params.h:4); rowstrides must be multiples of the block.
+ *
+ * ERRORS.  Every function returns an mgx_status; mgx_last_error() gives a
+ * message.  The reference aborts instead (cusuc.h:7-15, mpz-realloc.cu:5-11).
+ *
+ * THREADING.  Calls are stream-ordered and asynchronous unless stated; a
+ * call that must return a host value synchronizes its stream.  The library
+ * is thread-compatible: concurrent calls from several host threads are safe
+ * when they use different streams and different matrices.
+ */
+#ifndef MGX_H
+#define MGX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MGX_INTERLEAVE_BLOCK 256
+
+typedef uint64_t mgx_limb;
+
+typedef enum {
+    MGX_OK = 0,
+    MGX_ERR_CAPACITY = 1,    /* a value needs more than cap limbs */
+    MGX_ERR_SHAPE = 2,       /* sizes, strides or alignment not supported */
+    MGX_ERR_HIP = 3,         /* a HIP runtime call failed */
+    MGX_ERR_OOM = 4,         /* device allocation failed */
+    MGX_ERR_ARG = 5,         /* invalid argument (null pointer, bad enum) */
+    MGX_ERR_UNSUPPORTED = 6, /* not built for this device or configuration */
+    MGX_ERR_PARSE = 7        /* malformed text input */
+} mgx_status;
+
+/* ---------------- library ---------------- */
+int mgx_init(int device);
+const char *mgx_last_error(void);
+const char *mgx_version(void);
+/* Synchronizes the stream and reports (then clears) errors that kernels
+ * flagged on the device since the last check: MGX_ERR_CAPACITY when an
+ * output needed more than cap limbs.  The launchers are asynchronous and
+ * report through this; matrix-level calls check it themselves. */
+int mgx_check_error(void *stream);
+
+/* Device memory and stream helpers (so that C, ctypes or cgo callers need no
+ * other HIP binding). */
+int mgx_malloc(void **p, size_t bytes);
+int mgx_free(void *p);
+int mgx_memcpy_htod(void *dst, const void *src, size_t bytes, void *stream);
+int mgx_memcpy_dtoh(void *dst, const void *src, size_t bytes, void *stream);
+int mgx_memset(void *dst, int value, size_t bytes, void *stream);
+int mgx_stream_create(void **stream);
+int mgx_stream_destroy(void *stream);
+int mgx_stream_sync(void *stream);
+int mgx_device_sync(void);
+/* Elapsed milliseconds between two events recorded on a stream. */
+int mgx_event_create(void **ev);
+int mgx_event_destroy(void *ev);
+int mgx_event_record(void *ev, void *stream);
+int mgx_event_elapsed(float *ms, void *start, void *stop);
+
+/* ---------------- host utilities (no device) ---------------- */
+/* Seeded entries: magnitude of nbits uniform bits (mpz_urandomb semantics,
+ * mat-cpu/mat.c:40) and a uniform sign (mat-cpu/mat.c:41), from a
+ * SplitMix64-seeded xoshiro256** stream shared by every path. */
+int mgx_random_matrix(mgx_limb *host, int64_t W, int64_t H, int64_t rowstride, int64_t cap,
+                      int64_t nbits, uint64_t seed);
+/* Hex text in mat-gpu's format (mat-gpu/mpz.cu:85-204). */
+int64_t mgx_entry_to_hex(char *buf, int64_t buflen, const mgx_limb *entry, int64_t cap);
+int mgx_entry_from_hex(mgx_limb *entry, int64_t cap, const char *text);
+
+/* ---------------- launchers (mat-gpu/mat-gpu-launchers.h:7-61) ---------------- */
+/* C (=|+=) A*B per output entry.  Replaces launcher_mul / launcher_addmul
+ * (mat-gpu-launchers.cu:5-63). */
+int mgx_launcher_mul(int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A, int64_t Ars,
+                     const mgx_limb *B, int64_t Brs, int64_t cap, void *stream);
+int mgx_launcher_addmul(int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A,
+                        int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap, void *stream);
+int mgx_launcher_mul_strided(int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A,
+                             int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap, void *stream);
+int mgx_launcher_addmul_strided(int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A,
+                                int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap, void *stream);
+/* C = A +- B elementwise, W x H.  Replaces launcher_add / launcher_sub
+ * (mat-gpu-launchers.cu:65-147). */
+int mgx_launcher_add(int64_t W, int64_t H, mgx_limb *C, int64_t Crs, const mgx_limb *A, int64_t Ars,
+                     const mgx_limb *B, int64_t Brs, int64_t cap, void *stream);
+int mgx_launcher_sub(int64_t W, int64_t H, mgx_limb *C, int64_t Crs, const mgx_limb *A, int64_t Ars,
+                     const mgx_limb *B, int64_t Brs, int64_t cap, void *stream);
+int mgx_launcher_add_strided(int64_t W, int64_t H, mgx_limb *C, int64_t Crs, const mgx_limb *A, int64_t Ars,
+                             const mgx_limb *B, int64_t Brs, int64_t cap, void *stream);
+int mgx_launcher_sub_strided(int64_t W, int64_t H, mgx_limb *C, int64_t Crs, const mgx_limb *A, int64_t Ars,
+                             const mgx_limb *B, int64_t Brs, int64_t cap, void *stream);
+/* Interleave / de-interleave whole blocks of MGX_INTERLEAVE_BLOCK entries in
+ * place on the device (mat-gpu/main.cu:137-156 with atomsz = 8). */
+int mgx_interleave(mgx_limb *M, int64_t nentries, int64_t cap, void *stream);
+int mgx_deinterleave(mgx_limb *M, int64_t nentries, int64_t cap, void *stream);
+
+/* ---------------- matrix-level plain route (mat-gpu/mat-gpu.h:6-84) ---------------- */
+int mgx_add_matrices(int64_t W, int64_t H, mgx_limb *C, int64_t Crs, const mgx_limb *A, int64_t Ars,
+                     const mgx_limb *B, int64_t Brs, int64_t cap, void *stream);
+int mgx_sub_matrices(int64_t W, int64_t H, mgx_limb *C, int64_t Crs, const mgx_limb *A, int64_t Ars,
+                     const mgx_limb *B, int64_t Brs, int64_t cap, void *stream);
+int mgx_add_matrices_strided(int64_t W, int64_t H, mgx_limb *C, int64_t Crs, const mgx_limb *A, int64_t Ars,
+                             const mgx_limb *B, int64_t Brs, int64_t cap, void *stream);
+int mgx_sub_matrices_strided(int64_t W, int64_t H, mgx_limb *C, int64_t Crs, const mgx_limb *A, int64_t Ars,
+                             const mgx_limb *B, int64_t Brs, int64_t cap, void *stream);
+int mgx_naive_smallrect(int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A,
+                        int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap, void *stream);
+
+/* Square N x N recursion over 2x2 blocks down to the leaf size (the
+ * reference's BASE_SIZE = 256, mat-gpu/mat-params.h:5; here a tunable,
+ * mgx_set_param(MGX_PARAM_LEAF, ...)).  Temporaries come from a
+ * stream-ordered pool; no host synchronization inside. */
+typedef enum {
+    MGX_REC_NAIVE = 0,          /* naive_recursive_gpu (mat-gpu.cu:79-164) */
+    MGX_REC_STRASSEN = 1,       /* strassen_gpu (mat-gpu.cu:166-265) */
+    MGX_REC_WINOGRAD = 2,       /* winograd_gpu (mat-gpu.cu:267-315,367-420) */
+    MGX_REC_WINOGRAD_ADDMUL = 3 /* winograd_addmul_gpu (mat-gpu.cu:317-365,422-475) */
+} mgx_rec_method;
+int mgx_recursive(int method, int strided, int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A, int64_t Ars,
+                  const mgx_limb *B, int64_t Brs, int64_t cap, void *stream);
+int mgx_naive_recursive(int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A, int64_t Ars, const mgx_limb *B,
+                        int64_t Brs, int64_t cap, void *stream);
+int mgx_naive_recursive_strided(int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A, int64_t Ars,
+                                const mgx_limb *B, int64_t Brs, int64_t cap, void *stream);
+int mgx_strassen(int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A, int64_t Ars, const mgx_limb *B,
+                 int64_t Brs, int64_t cap, void *stream);
+int mgx_strassen_strided(int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A, int64_t Ars, const mgx_limb *B,
+                         int64_t Brs, int64_t cap, void *stream);
+int mgx_winograd(int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A, int64_t Ars, const mgx_limb *B,
+                 int64_t Brs, int64_t cap, void *stream);
+int mgx_winograd_strided(int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A, int64_t Ars, const mgx_limb *B,
+                         int64_t Brs, int64_t cap, void *stream);
+int mgx_winograd_addmul(int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A, int64_t Ars, const mgx_limb *B,
+                        int64_t Brs, int64_t cap, void *stream);
+int mgx_winograd_addmul_strided(int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A, int64_t Ars,
+                                const mgx_limb *B, int64_t Brs, int64_t cap, void *stream);
+/* Any K x M x N: square power-of-two-times-small blocks go to the recursion
+ * `method`, the rest is split as squarify_mul_pregpu does
+ * (mat-gpu/mat-pregpu.cu:469-576). */
+int mgx_squarify_mul(int method, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A,
+                     int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap, void *stream);
+
+/* Host-memory entry: copies A and B to the device, multiplies with the
+ * mat-gpu method code (1, 2, 3, 35, 37, 357, 4, 47 as in
+ * mat-gpu/main.cu:358-502; 8 = CRT-MFMA; 0 = cost-model choice), copies C
+ * back.  Replaces the *_pregpu layer (mat-gpu/mat-pregpu.h:6-63). */
+int mgx_matmul_host(int method, int64_t K, int64_t M, int64_t N, mgx_limb *C, const mgx_limb *A,
+                    const mgx_limb *B, int64_t cap, double *seconds);
+
+/* ---------------- multi-modular CRT route on INT8 MFMA (new; no reference counterpart) ---------------- */
+typedef struct mgx_crt_plan mgx_crt_plan;
+/* Plan for products with inner dimension K and |A| < 2^bitsA, |B| < 2^bitsB:
+ * the smallest prefix of the primes below 2^31 (descending) whose product
+ * exceeds 2^(bitsA + bitsB + ceil(log2 K) + 2), with all per-prime constants
+ * resident on the device.  Synchronous. */
+int mgx_crt_plan_create(mgx_crt_plan **plan, int64_t K, int64_t bitsA, int64_t bitsB);
+/* Largest bit length of |x| over a rows x cols matrix (synchronizes). */
+int mgx_crt_scan_bits(int64_t rows, int64_t cols, const mgx_limb *X, int64_t Xrs, int64_t cap, int64_t *bits,
+                      void *stream);
+int mgx_crt_plan_destroy(mgx_crt_plan *plan);
+int64_t mgx_crt_plan_primes(const mgx_crt_plan *plan);
+int mgx_crt_plan_prime_list(const mgx_crt_plan *plan, uint32_t *out);
+/* C = A*B through the plan (plan NULL: scan the inputs' bit lengths on the
+ * device, which synchronizes the stream once, and plan internally). */
+int mgx_crt_mfma_matmul(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs,
+                        const mgx_limb *A, int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap,
+                        void *stream);
+/* Stage kernels, exposed for testing and profiling.  Residues are planar
+ * int8 slices R[q][s][row][k] with row pitch kpad and rows padded to rpad
+ * (mgx_crt_residue_dims); Y is the per-prime output C mod p_q as
+ * [q][K*N] uint32 in [0, p_q). */
+int mgx_crt_residue_dims(const mgx_crt_plan *plan, int64_t rows, int64_t K, int64_t *rpad, int64_t *kpad);
+int mgx_crt_reduce(const mgx_crt_plan *plan, int transpose, int64_t rows, int64_t K, const mgx_limb *X,
+                   int64_t Xrs, int64_t cap, int64_t nbytes, int8_t *R, void *stream);
+int mgx_crt_residue_gemm(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t N, const int8_t *RA,
+                         const int8_t *RB, uint32_t *Y, void *stream);
+int mgx_crt_reconstruct(const mgx_crt_plan *plan, int64_t K, int64_t N, const uint32_t *Y, mgx_limb *C,
+                        int64_t Crs, int64_t cap, void *stream);
+
+/* ---------------- kept reference CRT API (mat-gpu/gpu_crt.h:6-14; host) ---------------- */
+/* Same arguments and meaning as gpu_crt_* on entries of capacity cap, using
+ * this library's own arithmetic (no GMP).  make_split_tree allocates the
+ * four arrays (cap+1 words per entry) with malloc; the caller frees them
+ * (mat-gpu/gpu_crt.cu:90-120).  Returns the tree length, or -status. */
+int mgx_crt_split(mgx_limb *m, mgx_limb *n, mgx_limb *mf, mgx_limb *nf, const mgx_limb *x,
+                  const mgx_limb *const *list, int64_t listlen, int64_t cap);
+int mgx_crt_merge(mgx_limb *x, const mgx_limb *a, const mgx_limb *b, const mgx_limb *m, const mgx_limb *n,
+                  const mgx_limb *mf, const mgx_limb *nf, int64_t cap);
+int mgx_crt_make_split_tree(mgx_limb **ms, mgx_limb **ns, mgx_limb **mfs, mgx_limb **nfs, const mgx_limb *x,
+                            uint64_t limit, int64_t cap);
+int mgx_crt_split_list(mgx_limb *res, const mgx_limb *x, const mgx_limb *ms, const mgx_limb *ns, int length,
+                       int64_t cap);
+int mgx_crt_merge_list(mgx_limb *x, const mgx_limb *split, const mgx_limb *ms, const mgx_limb *ns,
+                       const mgx_limb *mfs, const mgx_limb *nfs, int length, int64_t cap);
+
+/* ---------------- tunables ---------------- */
+typedef enum {
+    MGX_PARAM_LEAF = 0,          /* recursion leaf size (reference BASE_SIZE = 256) */
+    MGX_PARAM_TOOM22 = 1,        /* Toom-22 block cutoff in 64-limb blocks (0 = never) */
+    MGX_PARAM_WINOGRAD_DEPTH = 2 /* max recursion levels for the plain route (-1 = to the leaf) */
+} mgx_param;
+int mgx_set_param(int param, int64_t value);
+int64_t mgx_get_param(int param);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

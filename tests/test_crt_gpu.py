@@ -1,0 +1,179 @@
+"""Parity of the INT8-MFMA multi-modular route with the oracle (bit-exact)."""
+import random
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import entry_int, is_normalized, matmul, rand_ints, to_entries, to_ints
+
+pytestmark = pytest.mark.gpu
+
+
+def run_crt(mgx, A, B, cap, plan=None):
+    K, M = A.shape[:2]
+    N = B.shape[1]
+    dA, dB = mgx.DeviceBuffer.from_array(A), mgx.DeviceBuffer.from_array(B)
+    dC = mgx.DeviceBuffer(max(K * N, 1) * (cap + 1) * 8)
+    dC.zero()
+    mgx.call("mgx_crt_mfma_matmul", plan, K, M, N, dC.ptr, N, dA.ptr, M, dB.ptr, N, cap, None)
+    return dC.to_array((K, N, cap + 1))
+
+
+def product_plan_M(primes):
+    M = 1
+    for p in primes:
+        M *= p
+    return M
+
+
+@pytest.mark.parametrize("transpose", [0, 1])
+def test_reduce_kernel(gpu, transpose):
+    mgx = gpu
+    rows, K, bits, cap = 70, 45, 700, 14
+    vals = rand_ints(rows, K, bits, 3 + transpose)
+    vals[0][0], vals[1][1], vals[2][2] = 0, (1 << bits) - 1, -((1 << bits) - 1)
+    X = to_entries(vals if not transpose else [list(r) for r in zip(*vals)], cap)
+    plan = mgx.crt_plan(K, bits, bits)
+    try:
+        ps = mgx.crt_plan_primes(plan)
+        rp, kp = mgx.c_i64(), mgx.c_i64()
+        mgx.call("mgx_crt_residue_dims", plan, rows, K, mgx.ctypes.byref(rp), mgx.ctypes.byref(kp))
+        rpad, kpad = rp.value, kp.value
+        dX = mgx.DeviceBuffer.from_array(X)
+        dR = mgx.DeviceBuffer(4 * len(ps) * rpad * kpad)
+        xrs = X.shape[1]
+        mgx.call("mgx_crt_reduce", plan, transpose, rows, K, dX.ptr, xrs, cap, 0, dR.ptr, None)
+        mgx.call("mgx_check_error", None)
+        R = dR.to_array((len(ps), 4, rpad, kpad), dtype=np.int8)
+        for qi in [0, 1, len(ps) // 2, len(ps) - 1]:
+            p = ps[qi]
+            for r in range(rows):
+                for k in range(K):
+                    res = vals[r][k] % p
+                    cen = res - p if res > p // 2 else res
+                    assert [int(R[qi, s, r, k]) for s in range(4)] == oracle.slices(cen), (qi, r, k)
+            assert not R[qi, :, rows:, :].any() and not R[qi, :, :, K:].any()   # zero padding
+    finally:
+        mgx.call("mgx_crt_plan_destroy", plan)
+
+
+def test_residue_gemm_kernel(gpu):
+    mgx = gpu
+    K, M, N, bits, cap = 37, 50, 29, 260, 10
+    A, B = rand_ints(K, M, bits, 21), rand_ints(M, N, bits, 22)
+    plan = mgx.crt_plan(M, bits, bits)
+    try:
+        ps = mgx.crt_plan_primes(plan)
+        Mprod = product_plan_M(ps)
+        rp, kp, rpb = mgx.c_i64(), mgx.c_i64(), mgx.c_i64()
+        mgx.call("mgx_crt_residue_dims", plan, K, M, mgx.ctypes.byref(rp), mgx.ctypes.byref(kp))
+        mgx.call("mgx_crt_residue_dims", plan, N, M, mgx.ctypes.byref(rpb), None)
+        a, b = to_entries(A, cap), to_entries(B, cap)
+        dA, dB = mgx.DeviceBuffer.from_array(a), mgx.DeviceBuffer.from_array(b)
+        dRA = mgx.DeviceBuffer(4 * len(ps) * rp.value * kp.value)
+        dRB = mgx.DeviceBuffer(4 * len(ps) * rpb.value * kp.value)
+        ystride = (K * N + 63) // 64 * 64
+        dY = mgx.DeviceBuffer(len(ps) * ystride * 4)
+        mgx.call("mgx_crt_reduce", plan, 0, K, M, dA.ptr, M, cap, 0, dRA.ptr, None)
+        mgx.call("mgx_crt_reduce", plan, 1, N, M, dB.ptr, N, cap, 0, dRB.ptr, None)
+        mgx.call("mgx_crt_residue_gemm", plan, K, M, N, dRA.ptr, dRB.ptr, dY.ptr, None)
+        mgx.call("mgx_check_error", None)
+        Y = dY.to_array((len(ps), ystride), dtype=np.uint32)
+        C = matmul(A, B)
+        for qi in [0, 3, len(ps) - 1]:
+            p = ps[qi]
+            inv = pow((Mprod // p) % p, p - 2, p)
+            for i in range(K):
+                for j in range(N):
+                    w = int(Y[qi, i * N + j])
+                    d = [((w >> (8 * s)) & 255) - (256 if (w >> (8 * s)) & 128 else 0) for s in range(4)]
+                    y = sum(t * 256 ** s for s, t in enumerate(d)) % p
+                    assert y == (C[i][j] % p) * inv % p, (qi, i, j)
+    finally:
+        mgx.call("mgx_crt_plan_destroy", plan)
+
+
+SHAPES = [(1, 1, 1), (1, 5, 1), (3, 4, 5), (7, 1, 9), (33, 17, 65), (64, 64, 64), (130, 70, 129), (5, 300, 6)]
+
+
+@pytest.mark.parametrize("K,M,N", SHAPES)
+@pytest.mark.parametrize("bits", [1, 64, 200, 1024])
+def test_crt_matmul_small(gpu, K, M, N, bits):
+    if bits == 1024 and K * M * N > 64 * 64 * 64:
+        pytest.skip("oracle time")
+    A, B = rand_ints(K, M, bits, K * 7 + bits), rand_ints(M, N, bits, N * 11 + bits)
+    cap = (2 * bits + 12 + 63) // 64 + 1
+    C = run_crt(gpu, to_entries(A, cap), to_entries(B, cap), cap)
+    gpu.call("mgx_check_error", None)
+    assert to_ints(C) == matmul(A, B)
+    assert is_normalized(C)
+
+
+def test_crt_edge_values(gpu):
+    # zeros, +-1, all-ones magnitudes, exact cancellation to zero, one-sided signs
+    bits, cap = 2048, 66
+    top = (1 << bits) - 1
+    A = [[top, -top, 0, 1], [-1, 0, top, top], [0, 0, 0, 0]]
+    B = [[top, 1, 0], [top, -1, 5], [7, 0, -top], [-7, 0, top]]
+    C = run_crt(gpu, to_entries(A, cap), to_entries(B, cap), cap)
+    assert to_ints(C) == matmul(A, B)
+    assert is_normalized(C)
+
+
+def test_crt_long_inner_dimension(gpu):
+    # K beyond one int32 accumulation chunk (8192) takes the folding variant
+    K, M, N, bits, cap = 3, 8300, 2, 100, 6
+    A, B = rand_ints(K, M, bits, 1), rand_ints(M, N, bits, 2)
+    C = run_crt(gpu, to_entries(A, cap), to_entries(B, cap), cap)
+    assert to_ints(C) == matmul(A, B)
+
+
+def test_crt_capacity_error(gpu):
+    mgx = gpu
+    A = [[(1 << 300) - 1]]
+    cap = 6   # the product needs 10 limbs
+    with pytest.raises(mgx.MgxError) as ei:
+        run_crt(mgx, to_entries(A, cap), to_entries(A, cap), cap)
+    assert ei.value.status == 1
+
+
+def test_crt_explicit_plan_reused(gpu):
+    mgx = gpu
+    plan = mgx.crt_plan(40, 500, 300)
+    try:
+        for seed in range(3):
+            A, B = rand_ints(9, 40, 500, seed), rand_ints(40, 11, 300, seed + 100)
+            C = run_crt(mgx, to_entries(A, 14), to_entries(B, 14), 14, plan=plan)
+            assert to_ints(C) == matmul(A, B)
+    finally:
+        mgx.call("mgx_crt_plan_destroy", plan)
+
+
+def test_crt_full_size_properties(gpu):
+    """n=1024, 2048-bit entries: an independent modulus, sampled exact entries."""
+    mgx = gpu
+    n, bits, cap = 1024, 2048, 66
+    A, B = mgx.random_matrix(n, n, cap, bits, 1), mgx.random_matrix(n, n, cap, bits, 2)
+    C = run_crt(mgx, A, B, cap)
+    assert is_normalized(C[:8])
+    q = 67108859   # largest prime below 2^26, not a CRT prime (those are above 2^30)
+
+    def mod_q(X):
+        r = np.zeros(X.shape[:2], dtype=np.uint64)
+        sz = X[..., cap].view(np.int64)
+        t64 = np.uint64(pow(2, 64, q))
+        for li in range(cap - 1, -1, -1):
+            limb = np.where(np.abs(sz) > li, X[..., li], np.uint64(0)) % np.uint64(q)
+            r = (r * t64 + limb) % np.uint64(q)
+        return np.where(sz < 0, (np.uint64(q) - r) % np.uint64(q), r).astype(np.int64)
+
+    am, bm = mod_q(A), mod_q(B)
+    want = np.zeros((n, n), dtype=np.int64)
+    for k0 in range(0, n, 256):   # keep partial sums below 2^63
+        want = (want + am[:, k0:k0 + 256] @ bm[k0:k0 + 256, :]) % q
+    assert (mod_q(C) == want).all()
+    rng = random.Random(5)
+    rows, cols = rng.sample(range(n), 3), rng.sample(range(n), 3)
+    sub = oracle.mat_mul(np.ascontiguousarray(A[rows]), np.ascontiguousarray(B[:, cols]), cap)
+    assert to_ints(sub) == [[entry_int(C[r, c]) for c in cols] for r in rows]
