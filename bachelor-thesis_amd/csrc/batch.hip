@@ -1,0 +1,417 @@
+// batch.hip -- many independent exact products as one captured hipGraph.
+//
+// Replaces the reference's host job scheduler (mat-gpu/job.cu:32-336: a host
+// DAG on up to four streams with a polled chunk pool, never called): every
+// product of the batch is planned once on the host, then each stage of each
+// route is ONE persistent launch whose workgroups pull tiles from a device
+// work queue (an atomic counter over the concatenated tile ranges of all
+// products of that stage).  Stages are ordered by the launches themselves;
+// the whole sequence, counter reset included (a fill kernel, no memset
+// nodes), is captured once and replayed with hipGraphLaunch.
+#include "crt_kernels.h"
+#include "plain_kernels.h"
+#include "../../include/mgx_sched.h"
+#include <algorithm>
+#include <vector>
+
+struct mgx_crt_plan;
+namespace mgx {
+int cached_plan(int64_t K, int64_t bitsA, int64_t bitsB, const mgx_crt_plan **out);
+struct PlanView {
+    const PrimeConst *pc;
+    const int8_t *cst, *mc;
+    const uint32_t *m32;
+    int64_t P, kb, Kr, rows4pad, Lbp, out_limbs;
+};
+int plan_view(const mgx_crt_plan *pl, PlanView *v);
+}
+
+namespace mgx {
+
+struct ReduceJob {
+    const PrimeConst *pc;
+    const int8_t *cst;
+    int P, nrb, kb, transpose;
+    const uint64_t *X;
+    int64_t Xrs, W, rows, K, kpad, nent, plane;
+    int8_t *R;
+};
+struct GemmJob {
+    const PrimeConst *pc;
+    const int8_t *RA, *RB;
+    int64_t planeA, planeB, kpad, rowsC, colsC, total, ystride;
+    int ntm, ntn;
+    uint32_t *Y;
+};
+struct ReconJob {
+    const PrimeConst *pc;
+    const int8_t *Mc;
+    const uint32_t *M32, *Y;
+    int P, Kr, nrb;
+    int64_t ystride, nent, colsC, Crs, W, cap, out_limbs;
+    uint64_t *C;
+};
+struct PlainJob {
+    int64_t K, M, N, Crs, Ars, Brs, W, cap;
+    uint64_t *C;
+    const uint64_t *A, *B;
+};
+
+struct Queue {
+    unsigned long long *counter;
+    const int64_t *prefix;   // njobs + 1 tile offsets
+    int njobs;
+};
+
+__device__ __forceinline__ bool next_item(const Queue &q, int &job, int64_t &tile)
+{
+    __shared__ int64_t s_item;
+    if (threadIdx.x == 0) s_item = (int64_t)atomicAdd(q.counter, 1ull);
+    __syncthreads();
+    const int64_t it = s_item;
+    __syncthreads();
+    if (it >= q.prefix[q.njobs]) return false;
+    int lo = 0, hi = q.njobs - 1;   // last job with prefix <= it
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (q.prefix[mid] <= it) lo = mid;
+        else hi = mid - 1;
+    }
+    job = lo;
+    tile = it - q.prefix[lo];
+    return true;
+}
+
+__global__ void reset_counters_kernel(unsigned long long *c, int n)
+{
+    if ((int)threadIdx.x < n) c[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(256) void batch_reduce_kernel(Queue q, const ReduceJob *jobs, unsigned *err)
+{
+    int j;
+    int64_t t;
+    while (next_item(q, j, t)) {
+        const ReduceJob &b = jobs[j];
+        reduce_tile(t, b.pc, b.cst, b.P, b.nrb, b.kb, b.X, b.Xrs, b.W, b.transpose, b.rows, b.K, b.kpad, b.nent, b.R,
+                    b.plane, err);
+        __syncthreads();
+    }
+}
+
+template <bool CHUNKED>
+__global__ __launch_bounds__(256, 1) void batch_gemm_kernel(Queue q, const GemmJob *jobs)
+{
+    int j;
+    int64_t t;
+    while (next_item(q, j, t)) {
+        const GemmJob &g = jobs[j];
+        gemm_tile<CHUNKED>(t, g.pc, g.RA, g.RB, g.planeA, g.planeB, g.kpad, g.rowsC, g.colsC, g.ntm, g.ntn, g.total,
+                           g.Y, g.ystride);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void batch_reconstruct_kernel(Queue q, const ReconJob *jobs, unsigned *err)
+{
+    int j;
+    int64_t t;
+    while (next_item(q, j, t)) {
+        const ReconJob &r = jobs[j];
+        reconstruct_tile(t, r.pc, r.P, r.Mc, r.Kr, r.nrb, r.M32, r.Y, r.ystride, r.nent, r.colsC, r.C, r.Crs, r.W,
+                         r.cap, r.out_limbs, err);
+        __syncthreads();
+    }
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) void batch_plain_kernel(Queue q, const PlainJob *jobs, int toom, unsigned *err)
+{
+    int j;
+    int64_t t;
+    while (next_item(q, j, t)) {
+        const PlainJob &p = jobs[j];
+        plain_entries<NB, false, false>(t, p.K, p.M, p.N, p.C, p.Crs, p.A, p.Ars, p.B, p.Brs, p.W, p.cap, toom, err);
+    }
+}
+
+}  // namespace mgx
+
+using namespace mgx;
+
+struct mgx_batch {
+    std::vector<int> routes;
+    void *scratch = nullptr;       // residues, Y, job tables, prefixes, counters
+    int64_t scratch_bytes = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+const int NB_CLASSES[] = {1, 2, 3, 4, 6, 8};
+const int NCLASS = 6;
+
+int64_t clog2i(int64_t k)
+{
+    int64_t l = 0;
+    while ((1ll << l) < k) l++;
+    return l;
+}
+
+struct Layout {
+    int64_t off = 0;
+    int64_t take(int64_t bytes)
+    {
+        int64_t o = off;
+        off += round_up(std::max<int64_t>(bytes, 1), 256);
+        return o;
+    }
+};
+
+template <int NB>
+void launch_plain(int grid, hipStream_t s, Queue q, const PlainJob *jobs, int toom, unsigned *err)
+{
+    hipLaunchKernelGGL(batch_plain_kernel<NB>, dim3(grid), dim3(256), 0, s, q, jobs, toom, err);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, void *stream)
+{
+    MGX_TRY(ensure_init());
+    if (!out || (count && !prods)) return fail(MGX_ERR_ARG, "null argument");
+    hipStream_t s = as_stream(stream);
+    auto *b = new mgx_batch();
+    b->routes.assign(count, MGX_ROUTE_CRT);
+    int st = MGX_OK;
+    std::vector<int64_t> bitsA(count), bitsB(count);
+    std::vector<PlanView> views(count);
+    for (int64_t i = 0; i < count && st == MGX_OK; i++) {
+        const mgx_product &p = prods[i];
+        if (p.K < 0 || p.M < 1 || p.N < 0 || p.cap < 1 || !p.A || !p.B || !p.C) {
+            st = fail(MGX_ERR_SHAPE, "product %lld: bad shape or pointer", (long long)i);
+            break;
+        }
+        bitsA[i] = p.bitsA;
+        bitsB[i] = p.bitsB;
+        if (bitsA[i] <= 0) st = mgx_crt_scan_bits(p.K, p.M, p.A, p.Ars, p.cap, &bitsA[i], stream);
+        if (st == MGX_OK && bitsB[i] <= 0) st = mgx_crt_scan_bits(p.M, p.N, p.B, p.Brs, p.cap, &bitsB[i], stream);
+        bitsA[i] = std::max<int64_t>(bitsA[i], 1);
+        bitsB[i] = std::max<int64_t>(bitsB[i], 1);
+        int r = p.route;
+        if (st == MGX_OK && r == MGX_ROUTE_AUTO) st = mgx_dispatch_choose(p.K, p.M, p.N, p.A, p.Ars, p.B, p.Brs, p.cap, &r);
+        // the batch runs plain products as one flat per-entry launch
+        if (r == MGX_ROUTE_WINOGRAD1 || r == MGX_ROUTE_WINOGRAD2) r = MGX_ROUTE_PLAIN;
+        if (r == MGX_ROUTE_PLAIN && (std::max(bitsA[i], bitsB[i]) + 63) / 64 * 2 > 64 * 8)
+            r = MGX_ROUTE_CRT;   // beyond the plain kernel's widest class
+        b->routes[i] = r;
+        if (st == MGX_OK && r == MGX_ROUTE_CRT) {
+            const mgx_crt_plan *pl = nullptr;
+            st = cached_plan(p.M, bitsA[i], bitsB[i], &pl);
+            if (st == MGX_OK) st = plan_view(pl, &views[i]);
+            if (st == MGX_OK) views[i].out_limbs = ceil_div(bitsA[i] + bitsB[i] + clog2i(p.M), 64);
+        }
+    }
+    if (st != MGX_OK) { delete b; return st; }
+
+    // ---- job tables (sizes first, then one scratch block, then real pointers)
+    std::vector<ReduceJob> rj;
+    std::vector<GemmJob> gj[2];
+    std::vector<ReconJob> cj;
+    std::vector<PlainJob> pj[NCLASS];
+    std::vector<int64_t> rp{0}, gp[2] = {{0}, {0}}, cp{0}, pp[NCLASS];
+    for (auto &v : pp) v.push_back(0);
+    struct Dims { int64_t rpadA, rpadB, kpad, ystride; };
+    std::vector<Dims> dims(count);
+    std::vector<int> pclass(count, -1), gclass(count, -1);
+    size_t nred = 0, ngem[2] = {0, 0}, nrec = 0, npl[NCLASS] = {0, 0, 0, 0, 0, 0};
+    for (int64_t i = 0; i < count; i++) {
+        const mgx_product &p = prods[i];
+        if (p.K == 0 || p.N == 0) continue;
+        if (b->routes[i] == MGX_ROUTE_CRT) {
+            Dims &d = dims[i];
+            d.rpadA = round_up(p.K, 128); d.rpadB = round_up(p.N, 128);
+            d.kpad = round_up(p.M, 64); d.ystride = round_up(p.K * p.N, 64);
+            gclass[i] = d.kpad > 8192;
+            nred += 2; ngem[gclass[i]]++; nrec++;
+        } else {
+            int64_t words = 2 * ((std::max(bitsA[i], bitsB[i]) + 63) / 64);
+            int k = 0;
+            while (k < NCLASS - 1 && 64 * NB_CLASSES[k] < words) k++;
+            pclass[i] = k;
+            npl[k]++;
+        }
+    }
+    Layout L;
+    std::vector<int64_t> o_ra(count), o_rb(count), o_y(count);
+    for (int64_t i = 0; i < count; i++) {
+        if (gclass[i] < 0) continue;
+        const Dims &d = dims[i];
+        o_ra[i] = L.take(4 * views[i].P * d.rpadA * d.kpad);
+        o_rb[i] = L.take(4 * views[i].P * d.rpadB * d.kpad);
+        o_y[i] = L.take(views[i].P * d.ystride * 4);
+    }
+    const int64_t o_ctr = L.take(64 * 8);
+    const int64_t o_rj = L.take((int64_t)(nred * sizeof(ReduceJob))), o_rp = L.take((int64_t)(nred + 1) * 8);
+    int64_t o_gj[2], o_gp[2], o_pj[NCLASS], o_pp[NCLASS];
+    for (int c = 0; c < 2; c++) { o_gj[c] = L.take((int64_t)(ngem[c] * sizeof(GemmJob))); o_gp[c] = L.take((int64_t)(ngem[c] + 1) * 8); }
+    const int64_t o_cj = L.take((int64_t)(nrec * sizeof(ReconJob))), o_cp = L.take((int64_t)(nrec + 1) * 8);
+    for (int c = 0; c < NCLASS; c++) { o_pj[c] = L.take((int64_t)(npl[c] * sizeof(PlainJob))); o_pp[c] = L.take((int64_t)(npl[c] + 1) * 8); }
+    b->scratch_bytes = L.off;
+    if (hipMalloc(&b->scratch, (size_t)b->scratch_bytes) != hipSuccess) {
+        b->scratch = nullptr;
+        mgx_batch_destroy(b);
+        return fail(MGX_ERR_OOM, "batch scratch of %lld bytes", (long long)L.off);
+    }
+    char *base = (char *)b->scratch;
+    for (int64_t i = 0; i < count; i++) {
+        const mgx_product &p = prods[i];
+        if (gclass[i] >= 0) {
+            const PlanView &v = views[i];
+            const Dims &d = dims[i];
+            const int nrb = (int)(v.rows4pad / 64);
+            for (int t = 0; t < 2; t++) {
+                ReduceJob r;
+                r.pc = v.pc; r.cst = v.cst; r.P = (int)v.P; r.nrb = nrb; r.kb = (int)v.kb; r.transpose = t;
+                r.X = (const uint64_t *)(t ? p.B : p.A); r.Xrs = t ? p.Brs : p.Ars; r.W = p.cap + 1;
+                r.rows = t ? p.N : p.K; r.K = p.M; r.kpad = d.kpad;
+                r.nent = (t ? d.rpadB : d.rpadA) * d.kpad; r.plane = r.nent;
+                r.R = (int8_t *)(base + (t ? o_rb[i] : o_ra[i]));
+                rj.push_back(r);
+                rp.push_back(rp.back() + ceil_div(r.nent, 128) * nrb);
+            }
+            GemmJob g;
+            g.pc = v.pc; g.RA = (const int8_t *)(base + o_ra[i]); g.RB = (const int8_t *)(base + o_rb[i]);
+            g.planeA = d.rpadA * d.kpad; g.planeB = d.rpadB * d.kpad; g.kpad = d.kpad; g.rowsC = p.K; g.colsC = p.N;
+            g.ntm = (int)(d.rpadA / 128); g.ntn = (int)(d.rpadB / 64); g.total = (int64_t)g.ntm * g.ntn * v.P;
+            g.ystride = d.ystride; g.Y = (uint32_t *)(base + o_y[i]);
+            gj[gclass[i]].push_back(g);
+            gp[gclass[i]].push_back(gp[gclass[i]].back() + g.total);
+            ReconJob c;
+            c.pc = v.pc; c.Mc = v.mc; c.M32 = v.m32; c.Y = g.Y; c.P = (int)v.P; c.Kr = (int)v.Kr;
+            c.nrb = (int)(v.Lbp / 128); c.ystride = d.ystride; c.nent = p.K * p.N; c.colsC = p.N; c.Crs = p.Crs;
+            c.W = p.cap + 1; c.cap = p.cap; c.out_limbs = v.out_limbs; c.C = (uint64_t *)p.C;
+            cj.push_back(c);
+            cp.push_back(cp.back() + ceil_div(c.nent, 64));
+        } else if (pclass[i] >= 0) {
+            PlainJob q;
+            q.K = p.K; q.M = p.M; q.N = p.N; q.Crs = p.Crs; q.Ars = p.Ars; q.Brs = p.Brs; q.W = p.cap + 1; q.cap = p.cap;
+            q.C = (uint64_t *)p.C; q.A = (const uint64_t *)p.A; q.B = (const uint64_t *)p.B;
+            pj[pclass[i]].push_back(q);
+            pp[pclass[i]].push_back(pp[pclass[i]].back() + ceil_div(p.K * p.N, 4));
+        }
+    }
+    auto up = [&](int64_t o, const void *src, size_t n) -> int {
+        if (n) MGX_HIP(hipMemcpy(base + o, src, n, hipMemcpyHostToDevice));
+        return MGX_OK;
+    };
+    st = up(o_rj, rj.data(), rj.size() * sizeof(ReduceJob));
+    if (st == MGX_OK) st = up(o_rp, rp.data(), rp.size() * 8);
+    for (int c = 0; c < 2 && st == MGX_OK; c++) {
+        st = up(o_gj[c], gj[c].data(), gj[c].size() * sizeof(GemmJob));
+        if (st == MGX_OK) st = up(o_gp[c], gp[c].data(), gp[c].size() * 8);
+    }
+    if (st == MGX_OK) st = up(o_cj, cj.data(), cj.size() * sizeof(ReconJob));
+    if (st == MGX_OK) st = up(o_cp, cp.data(), cp.size() * 8);
+    for (int c = 0; c < NCLASS && st == MGX_OK; c++) {
+        st = up(o_pj[c], pj[c].data(), pj[c].size() * sizeof(PlainJob));
+        if (st == MGX_OK) st = up(o_pp[c], pp[c].data(), pp[c].size() * 8);
+    }
+    if (st != MGX_OK) { mgx_batch_destroy(b); return st; }
+    const int64_t off_counters = o_ctr;
+
+    // ---- capture: reset counters, plain classes, reduce, gemm x2, reconstruct
+    unsigned long long *ctr = (unsigned long long *)(base + off_counters);
+    unsigned *err = device_error_word();
+    const int toom = (int)param(MGX_PARAM_TOOM22);
+    auto queue = [&](int slot, int64_t o_prefix, size_t njobs) {
+        Queue q;
+        q.counter = ctr + slot;
+        q.prefix = (const int64_t *)(base + o_prefix);
+        q.njobs = (int)njobs;
+        return q;
+    };
+    hipStream_t cs = nullptr;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) { mgx_batch_destroy(b); return fail(MGX_ERR_HIP, "stream"); }
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, cs, ctr, 64);
+        for (int c = 0; c < NCLASS; c++) {
+            if (pj[c].empty()) continue;
+            Queue q = queue(5 + c, o_pp[c], pj[c].size());
+            int grid = (int)std::min<int64_t>(pp[c].back(), 2048);
+            const PlainJob *jobs = (const PlainJob *)(base + o_pj[c]);
+            switch (NB_CLASSES[c]) {
+            case 1: launch_plain<1>(grid, cs, q, jobs, toom, err); break;
+            case 2: launch_plain<2>(grid, cs, q, jobs, toom, err); break;
+            case 3: launch_plain<3>(grid, cs, q, jobs, toom, err); break;
+            case 4: launch_plain<4>(grid, cs, q, jobs, toom, err); break;
+            case 6: launch_plain<6>(grid, cs, q, jobs, toom, err); break;
+            default: launch_plain<8>(grid, cs, q, jobs, toom, err); break;
+            }
+        }
+        if (!rj.empty()) {
+            int grid = (int)std::min<int64_t>(rp.back(), 1024);
+            hipLaunchKernelGGL(batch_reduce_kernel, dim3(grid), dim3(256), 0, cs, queue(0, o_rp, rj.size()),
+                               (const ReduceJob *)(base + o_rj), err);
+        }
+        for (int c = 0; c < 2; c++) {
+            if (gj[c].empty()) continue;
+            int grid = (int)std::min<int64_t>(gp[c].back(), 256);
+            if (c == 0)
+                hipLaunchKernelGGL(batch_gemm_kernel<false>, dim3(grid), dim3(256), 0, cs, queue(1, o_gp[0], gj[0].size()),
+                                   (const GemmJob *)(base + o_gj[0]));
+            else
+                hipLaunchKernelGGL(batch_gemm_kernel<true>, dim3(grid), dim3(256), 0, cs, queue(2, o_gp[1], gj[1].size()),
+                                   (const GemmJob *)(base + o_gj[1]));
+        }
+        if (!cj.empty()) {
+            int grid = (int)std::min<int64_t>(cp.back(), 768);
+            hipLaunchKernelGGL(batch_reconstruct_kernel, dim3(grid), dim3(256), 0, cs, queue(3, o_cp, cj.size()),
+                               (const ReconJob *)(base + o_cj), err);
+        }
+        e = hipStreamEndCapture(cs, &b->graph);
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    hipStreamDestroy(cs);
+    if (e != hipSuccess) {
+        mgx_batch_destroy(b);
+        return fail(MGX_ERR_HIP, "batch graph capture: %s", hipGetErrorString(e));
+    }
+    (void)s;
+    *out = b;
+    return MGX_OK;
+}
+
+int mgx_batch_run(mgx_batch *b, void *stream)
+{
+    if (!b) return fail(MGX_ERR_ARG, "null batch");
+    MGX_HIP(hipGraphLaunch(b->exec, as_stream(stream)));
+    return MGX_OK;
+}
+
+int mgx_batch_route(const mgx_batch *b, int64_t i)
+{
+    if (!b || i < 0 || i >= (int64_t)b->routes.size()) return -1;
+    return b->routes[i];
+}
+
+int64_t mgx_batch_scratch_bytes(const mgx_batch *b) { return b ? b->scratch_bytes : -1; }
+
+int mgx_batch_destroy(mgx_batch *b)
+{
+    if (!b) return MGX_OK;
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->scratch) {
+        hipDeviceSynchronize();
+        hipFree(b->scratch);
+    }
+    delete b;
+    return MGX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,104 @@
+/*
+ * mgx_sched.h -- scheduling side of the library: seeded device inputs, the
+ * route autotuner, the learned cost model and the batched executor.
+ *
+ * None of these has a counterpart in the reference, whose job scheduler
+ * (mat-gpu/job.cu:32-336) is never called; the batched executor replaces it
+ * with a device work queue feeding persistent kernels.  The scheduling
+ * choices never change the arithmetic: every route is exact.
+ */
+#ifndef MGX_SCHED_H
+#define MGX_SCHED_H
+
+#include "mgx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- routes ---------------- */
+typedef enum {
+    MGX_ROUTE_CRT = 0,        /* INT8-MFMA multi-modular route */
+    MGX_ROUTE_PLAIN = 1,      /* per-entry mpn kernel, no recursion */
+    MGX_ROUTE_WINOGRAD1 = 2,  /* plain route, one Winograd level (squarify) */
+    MGX_ROUTE_WINOGRAD2 = 3,  /* plain route, two Winograd levels (squarify) */
+    MGX_ROUTES = 4,
+    MGX_ROUTE_AUTO = -1       /* cost model, falling back to the autotuner */
+} mgx_route;
+
+/* C = A*B on the device through one route.  Synchronous with status. */
+int mgx_matmul_route(int route, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A,
+                     int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap, void *stream);
+
+/* Device-side seeded entries (a counter-based SplitMix64 hash of seed, entry
+ * and limb): each entry has bits_hi bits with probability frac_hi, else
+ * bits_lo bits, magnitude uniform below 2^bits, sign uniform. */
+int mgx_random_matrix_device(mgx_limb *M, int64_t W, int64_t H, int64_t rowstride, int64_t cap, int64_t bits_lo,
+                             int64_t bits_hi, double frac_hi, uint64_t seed, void *stream);
+
+/* ---------------- features and the autotuner ---------------- */
+#define MGX_FEATURES 13
+/* Features of a product from its shape and the bit lengths of its entries
+ * (host-side histograms: counts of entries of A and B in the bit-length bins
+ * <=128, <=512, <=2048, <=8192, <=32768, larger). */
+int mgx_tune_features(int64_t K, int64_t M, int64_t N, const int64_t *hist6, double mean_bits, int64_t max_bits,
+                      double mean_blocks, float *out);
+/* Features of a device-resident product (scans the entries; synchronizes). */
+int mgx_tune_features_device(int64_t K, int64_t M, int64_t N, const mgx_limb *A, int64_t Ars, const mgx_limb *B,
+                             int64_t Brs, int64_t cap, float *out);
+/* Times every route on the given device matrices (C is scratch), median of
+ * reps runs after one warm-up; a plain route whose one-row estimate is
+ * hopeless against the CRT time gets that scaled estimate instead of a run
+ * and its flag set in *estimated (bit r).  Times in seconds. */
+int mgx_tune_time_routes(int64_t K, int64_t M, int64_t N, mgx_limb *C, const mgx_limb *A, const mgx_limb *B,
+                         int64_t cap, int reps, double *times, unsigned *estimated);
+
+/* ---------------- learned cost model (FP32 MLP, hand-written HIP kernels) ---------------- */
+typedef struct mgx_costmodel mgx_costmodel;
+int mgx_costmodel_create(mgx_costmodel **cm, int D, int H, int O, uint64_t seed);
+int mgx_costmodel_destroy(mgx_costmodel *cm);
+int64_t mgx_costmodel_param_count(const mgx_costmodel *cm);
+int mgx_costmodel_get_params(const mgx_costmodel *cm, float *out);
+int mgx_costmodel_set_params(mgx_costmodel *cm, const float *in);
+int mgx_costmodel_set_normalisation(mgx_costmodel *cm, const float *mean, const float *scale);
+int mgx_costmodel_loss_grad(mgx_costmodel *cm, const float *X, const float *T, const float *mask, int B,
+                            float *grad, float *loss);
+int mgx_costmodel_train(mgx_costmodel *cm, const float *X, const float *T, const float *mask, int B, int epochs,
+                        float lr, float *loss);
+int mgx_costmodel_predict(mgx_costmodel *cm, const float *X, int B, float *Y);
+/* Installs the model used by MGX_ROUTE_AUTO (NULL: none; the autotuner's
+ * exhaustive timing decides instead).  The library does not take ownership. */
+int mgx_dispatch_set_model(mgx_costmodel *cm);
+/* The route MGX_ROUTE_AUTO would take for a device-resident product. */
+int mgx_dispatch_choose(int64_t K, int64_t M, int64_t N, const mgx_limb *A, int64_t Ars, const mgx_limb *B,
+                        int64_t Brs, int64_t cap, int *route);
+
+/* ---------------- batched executor ---------------- */
+typedef struct {
+    int64_t K, M, N;            /* C is K x N, A is K x M, B is M x N */
+    mgx_limb *C;
+    int64_t Crs;
+    const mgx_limb *A;
+    int64_t Ars;
+    const mgx_limb *B;
+    int64_t Brs;
+    int64_t cap;
+    int64_t bitsA, bitsB;       /* bounds on |entries| (0: scanned at build) */
+    int route;                  /* mgx_route; MGX_ROUTE_AUTO lets the cost model choose */
+} mgx_product;
+typedef struct mgx_batch mgx_batch;
+/* Plans every product, allocates all scratch, builds the per-stage work
+ * queues and captures the whole batch (one persistent launch per stage per
+ * route class) as a hipGraph.  Synchronous. */
+int mgx_batch_create(mgx_batch **b, const mgx_product *products, int64_t count, void *stream);
+/* Replays the captured graph on the stream (asynchronous). */
+int mgx_batch_run(mgx_batch *b, void *stream);
+/* Route chosen for product i, and the number of graph nodes. */
+int mgx_batch_route(const mgx_batch *b, int64_t i);
+int64_t mgx_batch_scratch_bytes(const mgx_batch *b);
+int mgx_batch_destroy(mgx_batch *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
