@@ -71,6 +71,8 @@ int check_device_error(hipStream_t s)
         MGX_HIP(hipStreamSynchronize(s));
         if (h & DEV_ERR_CAPACITY) return fail(MGX_ERR_CAPACITY, "an output entry needs more limbs than cap");
         if (h & DEV_ERR_INPUT) return fail(MGX_ERR_SHAPE, "an input entry exceeds the plan's bit bound");
+        if (h & DEV_ERR_WIDTH)
+            return fail(MGX_ERR_UNSUPPORTED, "an operand is wider than the plain route's 512-word kernel");
         return fail(MGX_ERR_HIP, "device error word 0x%x", h);
     }
     return MGX_OK;

@@ -30,7 +30,7 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
     } while (0)
 
 // Device-side error word: kernels set bits, the host checks after a sync.
-enum : unsigned { DEV_ERR_CAPACITY = 1u, DEV_ERR_INPUT = 2u };
+enum : unsigned { DEV_ERR_CAPACITY = 1u, DEV_ERR_INPUT = 2u, DEV_ERR_WIDTH = 4u };
 unsigned *device_error_word();          // per-device, zero-initialised at init
 int check_device_error(hipStream_t s);  // synchronizes s, reads and clears the word
 

@@ -130,7 +130,9 @@ int launch_mul(int64_t K, int64_t M, int64_t N, uint64_t *C, int64_t Crs, const 
     if (S && (Crs % MGX_INTERLEAVE_BLOCK || Ars % MGX_INTERLEAVE_BLOCK || Brs % MGX_INTERLEAVE_BLOCK))
         return fail(MGX_ERR_SHAPE, "interleaved rowstrides must be multiples of %d", MGX_INTERLEAVE_BLOCK);
     if (K * N == 0) return MGX_OK;
-    int64_t nbneed = (opwords + 63) / 64;
+    // operands are at most cap limbs; wider than the widest kernel class (512
+    // words) the kernel itself reports the operands it cannot take
+    int64_t nbneed = std::min<int64_t>((opwords + 63) / 64, 8);
     dim3 g((unsigned)ceil_div(K * N, 4));
     unsigned *err = device_error_word();
     int toom = (int)param(MGX_PARAM_TOOM22);

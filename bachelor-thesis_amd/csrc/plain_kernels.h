@@ -267,7 +267,7 @@ __device__ __forceinline__ void plain_entries(int64_t blk, int64_t K, int64_t M,
         if (sa == 0 || sb == 0) continue;
         int64_t na = 2 * (sa < 0 ? -sa : sa), nb = 2 * (sb < 0 ? -sb : sb);
         int nba = (int)((na + 63) >> 6), nbb = (int)((nb + 63) >> 6);
-        if (nba > NB || nbb > NB) { if (l == 0) atomicOr(err, DEV_ERR_CAPACITY); continue; }
+        if (nba > NB || nbb > NB) { if (l == 0) atomicOr(err, DEV_ERR_WIDTH); continue; }
         uint32_t a[NB], b[NB];
 #pragma unroll
         for (int g = 0; g < NB; g++) {

@@ -202,7 +202,12 @@ int mgx_tune_time_routes(int64_t K, int64_t M, int64_t N, mgx_limb *C, const mgx
     if (reps < 1) reps = 1;
     unsigned est = 0;
     auto run = [&](int r, int64_t rows, double *out) -> int {
-        MGX_TRY(mgx_matmul_route(r, rows, M, N, C, N, A, M, B, N, cap, nullptr));   // warm-up (plans, pools)
+        int st0 = mgx_matmul_route(r, rows, M, N, C, N, A, M, B, N, cap, nullptr);   // warm-up (plans, pools)
+        if (st0 == MGX_ERR_UNSUPPORTED || st0 == MGX_ERR_CAPACITY) {   // route cannot take this product
+            *out = 1e30;
+            return MGX_OK;
+        }
+        MGX_TRY(st0);
         std::vector<double> t;
         for (int i = 0; i < reps; i++) {
             double t0 = now();
