@@ -14,24 +14,12 @@
 #include <algorithm>
 #include <vector>
 
-struct mgx_crt_plan;
-namespace mgx {
-int cached_plan(int64_t K, int64_t bitsA, int64_t bitsB, const mgx_crt_plan **out);
-struct PlanView {
-    const PrimeConst *pc;
-    const int8_t *cst, *mc;
-    const uint32_t *m32;
-    int64_t P, kb, Kr, rows4pad, Lbp, out_limbs;
-};
-int plan_view(const mgx_crt_plan *pl, PlanView *v);
-}
-
 namespace mgx {
 
 struct ReduceJob {
     const PrimeConst *pc;
     const int8_t *cst;
-    int P, nrb, kb, transpose;
+    int P, nrb, kb, transpose, resident;
     const uint64_t *X;
     int64_t Xrs, W, rows, K, kpad, nent, plane;
     int8_t *R;
@@ -47,9 +35,11 @@ struct ReconJob {
     const PrimeConst *pc;
     const int8_t *Mc;
     const uint32_t *M32, *Y;
-    int P, Kr, nrb;
-    int64_t ystride, nent, colsC, Crs, W, cap, out_limbs;
+    int P, Kr, nrb, nw;
+    int64_t ystride, nent, colsC, Crs, W, cap;
     uint64_t *C;
+    int64_t *Wv;
+    int32_t *gamma;
 };
 struct PlainJob {
     int64_t K, M, N, Crs, Ars, Brs, W, cap;
@@ -93,8 +83,12 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(Queue q, const Reduce
     int64_t t;
     while (next_item(q, j, t)) {
         const ReduceJob &b = jobs[j];
-        reduce_tile(t, b.pc, b.cst, b.P, b.nrb, b.kb, b.X, b.Xrs, b.W, b.transpose, b.rows, b.K, b.kpad, b.nent, b.R,
-                    b.plane, err);
+        if (b.resident)
+            reduce_tile_res(t, b.pc, b.cst, b.P, b.nrb, b.kb, b.X, b.Xrs, b.W, b.transpose, b.rows, b.K, b.kpad, b.R,
+                            b.plane, err);
+        else
+            reduce_tile(t, b.pc, b.cst, b.P, b.nrb, b.kb, b.X, b.Xrs, b.W, b.transpose, b.rows, b.K, b.kpad, b.nent,
+                        b.R, b.plane, err);
         __syncthreads();
     }
 }
@@ -112,14 +106,24 @@ __global__ __launch_bounds__(256, 1) void batch_gemm_kernel(Queue q, const GemmJ
     }
 }
 
-__global__ __launch_bounds__(256) void batch_reconstruct_kernel(Queue q, const ReconJob *jobs, unsigned *err)
+__global__ __launch_bounds__(256, 2) void batch_recon_gemm_kernel(Queue q, const ReconJob *jobs)
 {
     int j;
     int64_t t;
     while (next_item(q, j, t)) {
         const ReconJob &r = jobs[j];
-        reconstruct_tile(t, r.pc, r.P, r.Mc, r.Kr, r.nrb, r.M32, r.Y, r.ystride, r.nent, r.colsC, r.C, r.Crs, r.W,
-                         r.cap, r.out_limbs, err);
+        recon_gemm_tile(t, r.pc, r.P, r.Mc, r.Kr, r.nrb, r.nw, r.Y, r.ystride, r.nent, r.Wv, r.gamma);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void batch_recon_carry_kernel(Queue q, const ReconJob *jobs, unsigned *err)
+{
+    int j;
+    int64_t t;
+    while (next_item(q, j, t)) {
+        const ReconJob &r = jobs[j];
+        recon_carry_block(t, r.Wv, r.ystride, r.nw, r.gamma, r.M32, r.nent, r.colsC, r.C, r.Crs, r.W, r.cap, err);
         __syncthreads();
     }
 }
@@ -141,7 +145,7 @@ using namespace mgx;
 
 struct mgx_batch {
     std::vector<int> routes;
-    void *scratch = nullptr;       // residues, Y, job tables, prefixes, counters
+    void *scratch = nullptr;       // residues, Y, column sums, job tables, prefixes, counters
     int64_t scratch_bytes = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -151,13 +155,6 @@ namespace {
 
 const int NB_CLASSES[] = {1, 2, 3, 4, 6, 8};
 const int NCLASS = 6;
-
-int64_t clog2i(int64_t k)
-{
-    int64_t l = 0;
-    while ((1ll << l) < k) l++;
-    return l;
-}
 
 struct Layout {
     int64_t off = 0;
@@ -212,7 +209,6 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             const mgx_crt_plan *pl = nullptr;
             st = cached_plan(p.M, bitsA[i], bitsB[i], &pl);
             if (st == MGX_OK) st = plan_view(pl, &views[i]);
-            if (st == MGX_OK) views[i].out_limbs = ceil_div(bitsA[i] + bitsB[i] + clog2i(p.M), 64);
         }
     }
     if (st != MGX_OK) { delete b; return st; }
@@ -222,7 +218,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
     std::vector<GemmJob> gj[2];
     std::vector<ReconJob> cj;
     std::vector<PlainJob> pj[NCLASS];
-    std::vector<int64_t> rp{0}, gp[2] = {{0}, {0}}, cp{0}, pp[NCLASS];
+    std::vector<int64_t> rp{0}, gp[2] = {{0}, {0}}, cp{0}, kp{0}, pp[NCLASS];
     for (auto &v : pp) v.push_back(0);
     struct Dims { int64_t rpadA, rpadB, kpad, ystride; };
     std::vector<Dims> dims(count);
@@ -234,7 +230,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
         if (b->routes[i] == MGX_ROUTE_CRT) {
             Dims &d = dims[i];
             d.rpadA = round_up(p.K, 128); d.rpadB = round_up(p.N, 128);
-            d.kpad = round_up(p.M, 64); d.ystride = round_up(p.K * p.N, 64);
+            d.kpad = round_up(p.M, 64); d.ystride = crt_ystride(p.K * p.N);
             gclass[i] = d.kpad > 8192;
             nred += 2; ngem[gclass[i]]++; nrec++;
         } else {
@@ -246,19 +242,22 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
         }
     }
     Layout L;
-    std::vector<int64_t> o_ra(count), o_rb(count), o_y(count);
+    std::vector<int64_t> o_ra(count), o_rb(count), o_y(count), o_wv(count), o_ga(count);
     for (int64_t i = 0; i < count; i++) {
         if (gclass[i] < 0) continue;
         const Dims &d = dims[i];
         o_ra[i] = L.take(4 * views[i].P * d.rpadA * d.kpad);
         o_rb[i] = L.take(4 * views[i].P * d.rpadB * d.kpad);
         o_y[i] = L.take(views[i].P * d.ystride * 4);
+        o_wv[i] = L.take(views[i].nw * d.ystride * 8);
+        o_ga[i] = L.take(d.ystride * 4);
     }
     const int64_t o_ctr = L.take(64 * 8);
     const int64_t o_rj = L.take((int64_t)(nred * sizeof(ReduceJob))), o_rp = L.take((int64_t)(nred + 1) * 8);
     int64_t o_gj[2], o_gp[2], o_pj[NCLASS], o_pp[NCLASS];
     for (int c = 0; c < 2; c++) { o_gj[c] = L.take((int64_t)(ngem[c] * sizeof(GemmJob))); o_gp[c] = L.take((int64_t)(ngem[c] + 1) * 8); }
     const int64_t o_cj = L.take((int64_t)(nrec * sizeof(ReconJob))), o_cp = L.take((int64_t)(nrec + 1) * 8);
+    const int64_t o_kp = L.take((int64_t)(nrec + 1) * 8);
     for (int c = 0; c < NCLASS; c++) { o_pj[c] = L.take((int64_t)(npl[c] * sizeof(PlainJob))); o_pp[c] = L.take((int64_t)(npl[c] + 1) * 8); }
     b->scratch_bytes = L.off;
     if (hipMalloc(&b->scratch, (size_t)b->scratch_bytes) != hipSuccess) {
@@ -273,15 +272,17 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             const PlanView &v = views[i];
             const Dims &d = dims[i];
             const int nrb = (int)(v.rows4pad / 64);
+            const bool resident = v.kb <= RES_KB;
             for (int t = 0; t < 2; t++) {
                 ReduceJob r;
                 r.pc = v.pc; r.cst = v.cst; r.P = (int)v.P; r.nrb = nrb; r.kb = (int)v.kb; r.transpose = t;
+                r.resident = resident;
                 r.X = (const uint64_t *)(t ? p.B : p.A); r.Xrs = t ? p.Brs : p.Ars; r.W = p.cap + 1;
                 r.rows = t ? p.N : p.K; r.K = p.M; r.kpad = d.kpad;
                 r.nent = (t ? d.rpadB : d.rpadA) * d.kpad; r.plane = r.nent;
                 r.R = (int8_t *)(base + (t ? o_rb[i] : o_ra[i]));
                 rj.push_back(r);
-                rp.push_back(rp.back() + ceil_div(r.nent, 128) * nrb);
+                rp.push_back(rp.back() + ceil_div(r.nent, 128) * (resident ? 1 : nrb));
             }
             GemmJob g;
             g.pc = v.pc; g.RA = (const int8_t *)(base + o_ra[i]); g.RB = (const int8_t *)(base + o_rb[i]);
@@ -292,10 +293,12 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             gp[gclass[i]].push_back(gp[gclass[i]].back() + g.total);
             ReconJob c;
             c.pc = v.pc; c.Mc = v.mc; c.M32 = v.m32; c.Y = g.Y; c.P = (int)v.P; c.Kr = (int)v.Kr;
-            c.nrb = (int)(v.Lbp / 128); c.ystride = d.ystride; c.nent = p.K * p.N; c.colsC = p.N; c.Crs = p.Crs;
-            c.W = p.cap + 1; c.cap = p.cap; c.out_limbs = v.out_limbs; c.C = (uint64_t *)p.C;
+            c.nrb = (int)ceil_div(4 * v.nw, RECON_ROWS); c.nw = (int)v.nw; c.ystride = d.ystride;
+            c.nent = p.K * p.N; c.colsC = p.N; c.Crs = p.Crs; c.W = p.cap + 1; c.cap = p.cap; c.C = (uint64_t *)p.C;
+            c.Wv = (int64_t *)(base + o_wv[i]); c.gamma = (int32_t *)(base + o_ga[i]);
             cj.push_back(c);
-            cp.push_back(cp.back() + ceil_div(c.nent, 64));
+            cp.push_back(cp.back() + ceil_div(c.nent, RECON_ENT) * c.nrb);
+            kp.push_back(kp.back() + ceil_div(c.nent, CARRY_ENT));
         } else if (pclass[i] >= 0) {
             PlainJob q;
             q.K = p.K; q.M = p.M; q.N = p.N; q.Crs = p.Crs; q.Ars = p.Ars; q.Brs = p.Brs; q.W = p.cap + 1; q.cap = p.cap;
@@ -316,6 +319,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
     }
     if (st == MGX_OK) st = up(o_cj, cj.data(), cj.size() * sizeof(ReconJob));
     if (st == MGX_OK) st = up(o_cp, cp.data(), cp.size() * 8);
+    if (st == MGX_OK) st = up(o_kp, kp.data(), kp.size() * 8);
     for (int c = 0; c < NCLASS && st == MGX_OK; c++) {
         st = up(o_pj[c], pj[c].data(), pj[c].size() * sizeof(PlainJob));
         if (st == MGX_OK) st = up(o_pp[c], pp[c].data(), pp[c].size() * 8);
@@ -369,8 +373,11 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
                                    (const GemmJob *)(base + o_gj[1]));
         }
         if (!cj.empty()) {
-            int grid = (int)std::min<int64_t>(cp.back(), 768);
-            hipLaunchKernelGGL(batch_reconstruct_kernel, dim3(grid), dim3(256), 0, cs, queue(3, o_cp, cj.size()),
+            int grid = (int)std::min<int64_t>(cp.back(), 1024);
+            hipLaunchKernelGGL(batch_recon_gemm_kernel, dim3(grid), dim3(256), 0, cs, queue(3, o_cp, cj.size()),
+                               (const ReconJob *)(base + o_cj));
+            grid = (int)std::min<int64_t>(kp.back(), 1024);
+            hipLaunchKernelGGL(batch_recon_carry_kernel, dim3(grid), dim3(256), 0, cs, queue(4, o_kp, cj.size()),
                                (const ReconJob *)(base + o_cj), err);
         }
         e = hipStreamEndCapture(cs, &b->graph);

@@ -14,8 +14,10 @@
 //                   is written as four balanced int8 digits.
 //   3. reconstruct: explicit CRT, C = sum_q y_q (M/p_q) - beta M, again an int8
 //                   GEMM (the digits of (M/p_q) 256^s against the digits of
-//                   y_q) with the carry propagation fused into the epilogue;
-//                   beta comes from a 32-bit fixed-point sum of y_q/p_q.
+//                   y_q) whose epilogue leaves one signed 32-bit-word column
+//                   sum per word; beta comes from a 32-bit fixed-point sum of
+//                   y_q/p_q.  A carry pass per entry then finds the sign and
+//                   writes sign and magnitude.
 //
 // No floating point anywhere.  The reference has no counterpart (its
 // crt.cpp splits one modulus into a tree and is never called); the maths is
@@ -32,17 +34,20 @@
 namespace mgx {
 
 // ------------------------------------------------------------------------
-// bit-length scan: out[0] = max over entries of bit length of |x|
+// bit-length scan: out[0] = max over entries of bit length of |x|; one
+// atomic per workgroup
 // ------------------------------------------------------------------------
-__global__ void scan_bits_kernel(const uint64_t *X, int64_t rs, int64_t W, int64_t rows, int64_t cols, int *out)
+__global__ __launch_bounds__(256) void scan_bits_kernel(const uint64_t *X, int64_t rs, int64_t W, int64_t rows,
+                                                        int64_t cols, int *out)
 {
+    __shared__ int s_best[4];
     int64_t n = rows * cols;
     int best = 0;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
         int64_t r = e / cols, c = e % cols;
         const uint64_t *p = X + (r * rs + c) * W;
         int64_t s = (int64_t)p[W - 1];
-        int64_t a = s < 0 ? -s : s;
+        int64_t a = min(s < 0 ? -s : s, W - 1);
         while (a > 0 && p[a - 1] == 0) a--;
         if (a > 0) {
             int b = (int)((a - 1) * 64 + 64 - __clzll(p[a - 1]));
@@ -50,7 +55,12 @@ __global__ void scan_bits_kernel(const uint64_t *X, int64_t rs, int64_t W, int64
         }
     }
     for (int o = 32; o > 0; o >>= 1) best = max(best, __shfl_xor(best, o));
-    if ((threadIdx.x & 63) == 0) atomicMax(out, best);
+    if ((threadIdx.x & 63) == 0) s_best[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        best = max(max(s_best[0], s_best[1]), max(s_best[2], s_best[3]));
+        if (best > 0) atomicMax(out, best);
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -62,6 +72,14 @@ __global__ __launch_bounds__(256) void crt_reduce_kernel(
     int64_t kpad, int64_t nent, int8_t *__restrict__ R, int64_t plane, unsigned *__restrict__ err)
 {
     reduce_tile(blockIdx.x, pc, cst, P, nrb, kb, X, Xrs, W, transpose, rows, K, kpad, nent, R, plane, err);
+}
+
+__global__ __launch_bounds__(256) void crt_reduce_res_kernel(
+    const PrimeConst *__restrict__ pc, const int8_t *__restrict__ cst, int P, int nrb, int kb,
+    const uint64_t *__restrict__ X, int64_t Xrs, int64_t W, int transpose, int64_t rows, int64_t K,
+    int64_t kpad, int8_t *__restrict__ R, int64_t plane, unsigned *__restrict__ err)
+{
+    reduce_tile_res(blockIdx.x, pc, cst, P, nrb, kb, X, Xrs, W, transpose, rows, K, kpad, R, plane, err);
 }
 
 template <bool CHUNKED>
@@ -77,13 +95,24 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_kernel(
     gemm_tile<CHUNKED>(v, pc, RA, RB, planeA, planeB, kpad, rowsC, colsC, ntm, ntn, total, Y, ystride);
 }
 
-__global__ __launch_bounds__(256) void crt_reconstruct_kernel(
-    const PrimeConst *__restrict__ pc, int P, const int8_t *__restrict__ Mc, int Kr, int nrb,
-    const uint32_t *__restrict__ M32, const uint32_t *__restrict__ Y, int64_t ystride, int64_t nent,
-    int64_t colsC, uint64_t *__restrict__ C, int64_t Crs, int64_t W, int64_t cap, int64_t out_limbs,
-    unsigned *__restrict__ err)
+__global__ __launch_bounds__(256, 2) void crt_recon_gemm_kernel(
+    const PrimeConst *__restrict__ pc, int P, const int8_t *__restrict__ Mc, int Kr, int nrb, int nw,
+    const uint32_t *__restrict__ Y, int64_t ystride, int64_t nent, int64_t *__restrict__ Wv,
+    int32_t *__restrict__ gamma, int64_t total)
 {
-    reconstruct_tile(blockIdx.x, pc, P, Mc, Kr, nrb, M32, Y, ystride, nent, colsC, C, Crs, W, cap, out_limbs, err);
+    // XCD-aware order: the row blocks of one entry block share an L2
+    int64_t per = (total + 7) / 8;
+    int64_t v = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (v >= total) return;
+    recon_gemm_tile(v, pc, P, Mc, Kr, nrb, nw, Y, ystride, nent, Wv, gamma);
+}
+
+__global__ __launch_bounds__(256) void crt_recon_carry_kernel(
+    const int64_t *__restrict__ Wv, int64_t ystride, int nw, const int32_t *__restrict__ gamma,
+    const uint32_t *__restrict__ M32, int64_t nent, int64_t colsC, uint64_t *__restrict__ C, int64_t Crs, int64_t W,
+    int64_t cap, unsigned *__restrict__ err)
+{
+    recon_carry_block(blockIdx.x, Wv, ystride, nw, gamma, M32, nent, colsC, C, Crs, W, cap, err);
 }
 
 }  // namespace mgx
@@ -93,13 +122,10 @@ __global__ __launch_bounds__(256) void crt_reconstruct_kernel(
 // ========================================================================
 using namespace mgx;
 
-namespace mgx {
-int cached_plan(int64_t K, int64_t bitsA, int64_t bitsB, const mgx_crt_plan **out);
-}
-
 struct mgx_crt_plan {
     int64_t K, bitsA, bitsB, bound;   // |C| < 2^bound
     int64_t P, kb, Kr, rows4pad, Lbp, out_limbs;
+    int64_t nw;   // 32-bit words of the reconstruction's column sums kept (even)
     std::vector<uint32_t> primes;
     PrimeConst *d_pc = nullptr;
     int8_t *d_cst = nullptr;
@@ -230,9 +256,15 @@ int mgx_crt_plan_create(mgx_crt_plan **out, int64_t K, int64_t bitsA, int64_t bi
     pl->kb = round_up(ceil_div(std::max(bitsA, bitsB), 8), 64);
     pl->rows4pad = round_up(4 * P, 64);
     pl->Kr = round_up(4 * P, 64);
+    if (pl->Kr > 131072) {   // int32 column sums of 4P digit products must stay below 2^31
+        delete pl;
+        return fail(MGX_ERR_UNSUPPORTED, "%lld primes exceed the 32768-prime limit", (long long)P);
+    }
     int64_t mbytes = ceil_div(hb::bitlen(M), 8);
     pl->Lbp = round_up(mbytes + 1, 128);
     pl->out_limbs = ceil_div(pl->bound, 64);
+    // sum_q yc_q (M/p_q) has at most mbytes + 1 bytes; |C| at most out_limbs limbs
+    pl->nw = round_up(std::max(ceil_div(mbytes + 1, 4), 2 * pl->out_limbs), 2);
 
     // per-prime constants
     std::vector<PrimeConst> pc(P);
@@ -328,11 +360,17 @@ int mgx_crt_reduce(const mgx_crt_plan *pl, int transpose, int64_t rows, int64_t 
     mgx_crt_residue_dims(pl, rows, K, &rpad, &kpad);
     int64_t nent = rpad * kpad, plane = rpad * kpad;
     int nrb = (int)(pl->rows4pad / 64);
-    int64_t nblk = ceil_div(nent, 128) * nrb;
+    const bool resident = pl->kb <= RES_KB;   // the entries' bytes fit LDS: read them once
+    int64_t nblk = ceil_div(nent, 128) * (resident ? 1 : nrb);
     if (nblk > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "matrix too large for one reduce launch");
-    hipLaunchKernelGGL(crt_reduce_kernel, dim3((unsigned)nblk), dim3(256), 0, as_stream(stream), pl->d_pc,
-                       pl->d_cst, (int)pl->P, nrb, (int)pl->kb, (const uint64_t *)X, Xrs, cap + 1, transpose, rows,
-                       K, kpad, nent, R, plane, device_error_word());
+    if (resident)
+        hipLaunchKernelGGL(crt_reduce_res_kernel, dim3((unsigned)nblk), dim3(256), 0, as_stream(stream), pl->d_pc,
+                           pl->d_cst, (int)pl->P, nrb, (int)pl->kb, (const uint64_t *)X, Xrs, cap + 1, transpose,
+                           rows, K, kpad, R, plane, device_error_word());
+    else
+        hipLaunchKernelGGL(crt_reduce_kernel, dim3((unsigned)nblk), dim3(256), 0, as_stream(stream), pl->d_pc,
+                           pl->d_cst, (int)pl->P, nrb, (int)pl->kb, (const uint64_t *)X, Xrs, cap + 1, transpose,
+                           rows, K, kpad, nent, R, plane, device_error_word());
     MGX_HIP(hipGetLastError());
     return MGX_OK;
 }
@@ -347,7 +385,7 @@ int mgx_crt_residue_gemm(const mgx_crt_plan *pl, int64_t K, int64_t M, int64_t N
     int ntm = (int)(rpadA / 128), ntn = (int)(rpadB / 64);
     int64_t total = (int64_t)ntm * ntn * pl->P;
     int64_t grid = round_up(total, 8);
-    int64_t ystride = round_up(K * N, 64);
+    int64_t ystride = crt_ystride(K * N);
     // products with K beyond one chunk take the variant that folds between chunks
     if (kpad <= KCHUNK)
         hipLaunchKernelGGL(crt_gemm_kernel<false>, dim3((unsigned)grid), dim3(256), 0, as_stream(stream), pl->d_pc,
@@ -359,39 +397,85 @@ int mgx_crt_residue_gemm(const mgx_crt_plan *pl, int64_t K, int64_t M, int64_t N
     return MGX_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// Both reconstruct passes with caller-provided scratch: Wv holds nw x ystride
+// int64 words, gamma ystride int32.
+int reconstruct_with(const mgx_crt_plan *pl, int64_t K, int64_t N, const uint32_t *Y, mgx_limb *C, int64_t Crs,
+                     int64_t cap, int64_t *Wv, int32_t *gamma, hipStream_t s)
+{
+    const int64_t nent = K * N, ystride = crt_ystride(nent);
+    const int nrb = (int)ceil_div(4 * pl->nw, RECON_ROWS);
+    const int64_t total = ceil_div(nent, RECON_ENT) * nrb;
+    if (round_up(total, 8) > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "product too large for one reconstruct launch");
+    hipLaunchKernelGGL(crt_recon_gemm_kernel, dim3((unsigned)round_up(total, 8)), dim3(256), 0, s, pl->d_pc,
+                       (int)pl->P, pl->d_mc, (int)pl->Kr, nrb, (int)pl->nw, Y, ystride, nent, Wv, gamma, total);
+    MGX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(crt_recon_carry_kernel, dim3((unsigned)ceil_div(nent, CARRY_ENT)), dim3(256), 0, s, Wv,
+                       ystride, (int)pl->nw, gamma, pl->d_m32, nent, N, (uint64_t *)C, Crs, cap + 1, cap,
+                       device_error_word());
+    MGX_HIP(hipGetLastError());
+    return MGX_OK;
+}
+
+size_t recon_scratch_bytes(const mgx_crt_plan *pl, int64_t nent)
+{
+    return (size_t)(pl->nw * crt_ystride(nent) * 8 + crt_ystride(nent) * 4);
+}
+}  // namespace
+
+extern "C" {
+
 int mgx_crt_reconstruct(const mgx_crt_plan *pl, int64_t K, int64_t N, const uint32_t *Y, mgx_limb *C, int64_t Crs,
                         int64_t cap, void *stream)
 {
     if (!pl || !Y || !C) return fail(MGX_ERR_ARG, "null argument");
-    int64_t nent = K * N, ystride = round_up(K * N, 64);
-    int64_t grid = ceil_div(nent, 64);
-    unsigned *err = device_error_word();
-    hipLaunchKernelGGL(crt_reconstruct_kernel, dim3((unsigned)grid), dim3(256), 0, as_stream(stream), pl->d_pc,
-                       (int)pl->P, pl->d_mc, (int)pl->Kr, (int)(pl->Lbp / 128), pl->d_m32, Y, ystride, nent, N,
-                       (uint64_t *)C, Crs, cap + 1, cap, pl->out_limbs, err);
-    MGX_HIP(hipGetLastError());
+    if (K == 0 || N == 0) return MGX_OK;
+    hipStream_t s = as_stream(stream);
+    char *scr = nullptr;
+    MGX_TRY(scratch_alloc((void **)&scr, recon_scratch_bytes(pl, K * N), s));
+    int st = reconstruct_with(pl, K, N, Y, C, Crs, cap, (int64_t *)scr,
+                              (int32_t *)(scr + pl->nw * crt_ystride(K * N) * 8), s);
+    scratch_free(scr, s);
+    return st;
+}
+
+int64_t mgx_crt_y_stride(int64_t K, int64_t N) { return crt_ystride(K * N); }
+
+}  // extern "C"
+
+namespace {
+// Bit lengths of up to two matrices with one stream synchronization.
+int scan_bits_pair(int nmat, const int64_t *rows, const int64_t *cols, const mgx_limb *const *X, const int64_t *Xrs,
+                   int64_t cap, int64_t *bits, hipStream_t s)
+{
+    int *d = nullptr;
+    MGX_TRY(scratch_alloc((void **)&d, 2 * sizeof(int), s));
+    MGX_HIP(hipMemsetAsync(d, 0, 2 * sizeof(int), s));
+    for (int m = 0; m < nmat; m++) {
+        int64_t n = rows[m] * cols[m];
+        int grid = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(n, 256), 1), 1024);
+        if (n > 0)
+            hipLaunchKernelGGL(scan_bits_kernel, dim3(grid), dim3(256), 0, s, (const uint64_t *)X[m], Xrs[m], cap + 1,
+                               rows[m], cols[m], d + m);
+    }
+    int h[2] = {0, 0};
+    MGX_HIP(hipMemcpyAsync(h, d, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    MGX_TRY(scratch_free(d, s));
+    MGX_HIP(hipStreamSynchronize(s));
+    for (int m = 0; m < nmat; m++) bits[m] = h[m];
     return MGX_OK;
 }
+}  // namespace
+
+extern "C" {
 
 int mgx_crt_scan_bits(int64_t rows, int64_t cols, const mgx_limb *X, int64_t Xrs, int64_t cap, int64_t *bits,
                       void *stream)
 {
     MGX_TRY(ensure_init());
-    hipStream_t s = as_stream(stream);
-    int *d = nullptr;
-    MGX_TRY(scratch_alloc((void **)&d, sizeof(int), s));
-    MGX_HIP(hipMemsetAsync(d, 0, sizeof(int), s));
-    int64_t n = rows * cols;
-    int grid = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(n, 256), 1), 4096);
-    if (n > 0)
-        hipLaunchKernelGGL(scan_bits_kernel, dim3(grid), dim3(256), 0, s, (const uint64_t *)X, Xrs, cap + 1, rows,
-                           cols, d);
-    int h = 0;
-    MGX_HIP(hipMemcpyAsync(&h, d, sizeof(int), hipMemcpyDeviceToHost, s));
-    MGX_TRY(scratch_free(d, s));
-    MGX_HIP(hipStreamSynchronize(s));
-    *bits = h;
-    return MGX_OK;
+    return scan_bits_pair(1, &rows, &cols, &X, &Xrs, cap, bits, as_stream(stream));
 }
 
 int mgx_crt_mfma_matmul(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs,
@@ -408,31 +492,37 @@ int mgx_crt_mfma_matmul(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t 
     }
     const mgx_crt_plan *pl = plan;
     if (!pl) {
-        int64_t ba, bb;
-        MGX_TRY(mgx_crt_scan_bits(K, M, A, Ars, cap, &ba, stream));
-        MGX_TRY(mgx_crt_scan_bits(M, N, B, Brs, cap, &bb, stream));
-        MGX_TRY(cached_plan(M, ba, bb, &pl));
+        const int64_t rows[2] = {K, M}, cols[2] = {M, N}, rs[2] = {Ars, Brs};
+        const mgx_limb *X[2] = {A, B};
+        int64_t bits[2];
+        MGX_TRY(scan_bits_pair(2, rows, cols, X, rs, cap, bits, s));
+        MGX_TRY(cached_plan(M, bits[0], bits[1], &pl));
     } else if (plan->K < M) {
         return fail(MGX_ERR_SHAPE, "plan made for inner dimension %lld < %lld", (long long)plan->K, (long long)M);
     }
     int64_t rpadA, rpadB, kpad;
     mgx_crt_residue_dims(pl, K, M, &rpadA, &kpad);
     mgx_crt_residue_dims(pl, N, M, &rpadB, nullptr);
-    int8_t *RA = nullptr, *RB = nullptr;
-    uint32_t *Y = nullptr;
-    int st = MGX_OK;
-    size_t bytesA = (size_t)(4 * pl->P * rpadA * kpad), bytesB = (size_t)(4 * pl->P * rpadB * kpad);
-    size_t bytesY = (size_t)pl->P * round_up(K * N, 64) * 4;
-    if ((st = scratch_alloc((void **)&RA, bytesA, s)) == MGX_OK && (st = scratch_alloc((void **)&RB, bytesB, s)) == MGX_OK &&
-        (st = scratch_alloc((void **)&Y, bytesY, s)) == MGX_OK) {
-        if (st == MGX_OK) st = mgx_crt_reduce(pl, 0, K, M, A, Ars, cap, 0, RA, stream);
+    // one scratch block: residues of A and B, Y, then the reconstruction's
+    // column sums and gamma (which reuse the residue bytes when those are larger)
+    const size_t bytesA = (size_t)(4 * pl->P * rpadA * kpad), bytesB = (size_t)(4 * pl->P * rpadB * kpad);
+    const size_t bytesY = (size_t)pl->P * crt_ystride(K * N) * 4;
+    const size_t bytesR = recon_scratch_bytes(pl, K * N);
+    const size_t offY = (size_t)round_up((int64_t)(bytesA + bytesB), 256);
+    const size_t offR = bytesR <= offY ? 0 : (size_t)round_up((int64_t)(offY + bytesY), 256);
+    char *scr = nullptr;
+    int st = scratch_alloc((void **)&scr, std::max(offY + bytesY, offR + bytesR), s);
+    if (st == MGX_OK) {
+        int8_t *RA = (int8_t *)scr, *RB = (int8_t *)scr + bytesA;
+        uint32_t *Y = (uint32_t *)(scr + offY);
+        int64_t *Wv = (int64_t *)(scr + offR);
+        int32_t *gam = (int32_t *)(scr + offR + pl->nw * crt_ystride(K * N) * 8);
+        st = mgx_crt_reduce(pl, 0, K, M, A, Ars, cap, 0, RA, stream);
         if (st == MGX_OK) st = mgx_crt_reduce(pl, 1, N, M, B, Brs, cap, 0, RB, stream);
         if (st == MGX_OK) st = mgx_crt_residue_gemm(pl, K, M, N, RA, RB, Y, stream);
-        if (st == MGX_OK) st = mgx_crt_reconstruct(pl, K, N, Y, C, Crs, cap, stream);
+        if (st == MGX_OK) st = reconstruct_with(pl, K, N, Y, C, Crs, cap, Wv, gam, s);
+        scratch_free(scr, s);
     }
-    scratch_free(RA, s);
-    scratch_free(RB, s);
-    scratch_free(Y, s);
     if (st == MGX_OK) st = check_device_error(s);
     return st;
 }
@@ -440,17 +530,12 @@ int mgx_crt_mfma_matmul(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t 
 }  // extern "C"
 
 namespace mgx {
-struct PlanView {
-    const PrimeConst *pc;
-    const int8_t *cst, *mc;
-    const uint32_t *m32;
-    int64_t P, kb, Kr, rows4pad, Lbp, out_limbs;
-};
 int plan_view(const mgx_crt_plan *pl, PlanView *v)
 {
     v->pc = pl->d_pc; v->cst = pl->d_cst; v->mc = pl->d_mc; v->m32 = pl->d_m32;
     v->P = pl->P; v->kb = pl->kb; v->Kr = pl->Kr; v->rows4pad = pl->rows4pad; v->Lbp = pl->Lbp;
     v->out_limbs = pl->out_limbs;
+    v->nw = pl->nw;
     return MGX_OK;
 }
 

@@ -150,6 +150,105 @@ __device__ __forceinline__ void reduce_tile(int64_t blk,
         }
 }
 
+
+// Entry-resident reduce for entries of at most 256 bytes: one workgroup
+// stages its 128 entries in LDS once and sweeps every prime block over them;
+// the residues of a prime block leave through LDS as 16-byte stores of 128
+// consecutive entries per (prime, slice) plane.
+constexpr int RES_KB = 256;
+constexpr int RES_PITCH = RES_KB + 16;
+
+__device__ __forceinline__ void reduce_tile_res(int64_t blk,
+    const PrimeConst *__restrict__ pc, const int8_t *__restrict__ cst, int P, int nrb, int kb,
+    const uint64_t *__restrict__ X, int64_t Xrs, int64_t W, int transpose, int64_t rows, int64_t K,
+    int64_t kpad, int8_t *__restrict__ R, int64_t plane, unsigned *__restrict__ err)
+{
+    __shared__ __attribute__((aligned(16))) int8_t LE[128 * RES_PITCH];
+    __shared__ __attribute__((aligned(16))) int8_t LC[64 * RES_PITCH];
+    __shared__ __attribute__((aligned(16))) int8_t LO[64 * 128];
+    __shared__ int64_t s_base[128];   // entry offset in words, -1 for padding
+    __shared__ int s_nl[128];         // limbs to read, ~limbs when negative
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t e0 = blk * 128;
+    const int cpe = kb / 16;   // 16-byte chunks per entry
+
+    if (tid < 128) {
+        const int64_t e = e0 + tid, r = e / kpad, k = e % kpad;
+        int64_t base = -1;
+        int nl = 0;
+        bool neg = false;
+        if (r < rows && k < K) {
+            base = transpose ? (k * Xrs + r) * W : (r * Xrs + k) * W;
+            const int64_t s = (int64_t)X[base + W - 1];
+            const int64_t a = min(s < 0 ? -s : s, W - 1);
+            for (int64_t x = kb / 8; x < a; x++)   // limbs beyond the plan's bound must be zero
+                if (X[base + x]) { atomicOr(err, DEV_ERR_INPUT); break; }
+            nl = (int)min<int64_t>(a, kb / 8);
+            neg = s < 0;
+        }
+        s_base[tid] = base;
+        s_nl[tid] = neg ? ~nl : nl;
+    }
+    __syncthreads();
+    for (int c = tid; c < 128 * cpe; c += 256) {
+        const int el = c / cpe, part = c - el * cpe;
+        const int64_t base = s_base[el];
+        const int v = s_nl[el], nl = v < 0 ? ~v : v;
+        uint64_t v0 = 0, v1 = 0;
+        if (2 * part < nl) v0 = X[base + 2 * part];
+        if (2 * part + 1 < nl) v1 = X[base + 2 * part + 1];
+        uint64_t *d = (uint64_t *)(LE + el * RES_PITCH + 16 * part);
+        d[0] = v0 ^ 0x8080808080808080ull;   // byte - 128
+        d[1] = v1 ^ 0x8080808080808080ull;
+    }
+    const int el = wave * 32 + (lane & 31);
+    for (int rb = 0; rb < nrb; rb++) {
+        __syncthreads();   // LE staged; LC and LO free again
+        for (int c = tid; c < 64 * cpe; c += 256) {
+            const int row = c / cpe, part = c - row * cpe;
+            *(v4i *)(LC + row * RES_PITCH + 16 * part) =
+                *(const v4i *)(cst + (int64_t)(rb * 64 + row) * kb + 16 * part);
+        }
+        __syncthreads();
+        v16i acc[2] = {(v16i){0}, (v16i){0}};
+        for (int k32 = 0; k32 < kb / 32; k32++) {
+            const int ko = k32 * 32 + 16 * (lane >> 5);
+            const v4i bf = *(const v4i *)(LE + el * RES_PITCH + ko);
+#pragma unroll
+            for (int rt = 0; rt < 2; rt++) {
+                const v4i af = *(const v4i *)(LC + (rt * 32 + (lane & 31)) * RES_PITCH + ko);
+                acc[rt] = mfma32(af, bf, acc[rt]);
+            }
+        }
+        const bool real = s_base[el] >= 0, neg = s_nl[el] < 0;
+#pragma unroll
+        for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const int ql = rt * 8 + 2 * g + (lane >> 5), q = rb * 16 + ql;
+                uint32_t word = 0;
+                if (real && q < P) {
+                    const PrimeConst c = pc[q];
+                    const int64_t v = (int64_t)acc[rt][4 * g] + ((int64_t)acc[rt][4 * g + 1] << 8) +
+                                      ((int64_t)acc[rt][4 * g + 2] << 16) + ((int64_t)acc[rt][4 * g + 3] << 24) +
+                                      c.kofs;
+                    uint32_t res = mod_s64(v, c);
+                    if (neg && res) res = c.p - res;
+                    word = slice4(centre(res, c.p));
+                }
+#pragma unroll
+                for (int s = 0; s < 4; s++) LO[(4 * ql + s) * 128 + el] = (int8_t)(word >> (8 * s));
+            }
+        __syncthreads();
+        for (int c = tid; c < 64 * 8; c += 256) {
+            const int row = c >> 3, part = c & 7, q = rb * 16 + (row >> 2);
+            if (q < P)
+                *(v4i *)(R + ((int64_t)q * 4 + (row & 3)) * plane + e0 + 16 * part) =
+                    *(const v4i *)(LO + row * 128 + 16 * part);
+        }
+    }
+}
+
 // ------------------------------------------------------------------------
 // 2. residue GEMM.  Tile 128 (i) x 64 (j) per prime; 4 waves of 64 x 32.
 // ------------------------------------------------------------------------
@@ -270,196 +369,232 @@ __device__ __forceinline__ void gemm_tile(int64_t v,
 }
 
 // ------------------------------------------------------------------------
-// 3. reconstruct.  64 entries per block; row blocks of 128 result bytes.
+// 3. reconstruct, in two passes.
+//    (a) recon_gemm_tile: X' = sum_q yc_q (M/p_q), with yc_q the centred
+//        y_q, as an int8 GEMM of the digits of (M/p_q) 256^s (rows = result
+//        byte positions) against the digits of yc_q (columns = entries).  The
+//        epilogue folds each group of four byte rows into one signed 32-bit
+//        word value (|value| < 2^49) stored word-major, Wv[t][e].  The
+//        workgroups of the first row block also form the 32-bit fixed-point
+//        sum of y_q / p_q and the count of negative yc_q into
+//        gamma[e] = round(sum) - count, so that C = X' - gamma M.
+//    (b) recon_carry_block: one thread per entry finds the sign of
+//        X' - gamma M from its top words down, then runs the carry chain of
+//        |X' - gamma M| low to high; limbs leave through LDS so that each
+//        entry's limbs are stored as whole 128-byte lines.
 // ------------------------------------------------------------------------
-__device__ __forceinline__ void reconstruct_tile(int64_t blk,
-    const PrimeConst *__restrict__ pc, int P, const int8_t *__restrict__ Mc, int Kr, int nrb,
-    const uint32_t *__restrict__ M32, const uint32_t *__restrict__ Y, int64_t ystride, int64_t nent,
-    int64_t colsC, uint64_t *__restrict__ C, int64_t Crs, int64_t W, int64_t cap, int64_t out_limbs,
-    unsigned *__restrict__ err)
+constexpr int RECON_ROWS = 128;   // result bytes per workgroup
+constexpr int RECON_ENT = 256;    // entries per workgroup (64 per wave)
+
+__device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *__restrict__ pc, int P,
+    const int8_t *__restrict__ Mc, int Kr, int nrb, int nw, const uint32_t *__restrict__ Y, int64_t ystride,
+    int64_t nent, int64_t *__restrict__ Wv, int32_t *__restrict__ gamma)
 {
-    __shared__ __attribute__((aligned(16))) int8_t LA[128 * LDS_PITCH];
-    __shared__ __attribute__((aligned(16))) int8_t LB[64 * LDS_PITCH];
-    __shared__ int32_t LS[64][132];
-    __shared__ int64_t s_fix[4][64];
-    __shared__ int32_t s_cnt[4][64];
-    __shared__ int64_t s_carry[64];
-    __shared__ int32_t s_gamma[64];
-    __shared__ int64_t s_seg[64][4][2];    // local carry-out, first digit
-    __shared__ int32_t s_segf[64][4];      // bit0: digits 1..7 all ones, bit1: all zero
-    __shared__ int64_t s_cin[64][4];
-    __shared__ int32_t s_top[64];          // highest non-zero limb (two's complement words)
-    __shared__ int32_t s_topn[64];         // highest non-zero limb after negation
-    __shared__ int32_t s_flags[64];        // bit0: some limb >= cap non-zero, bit1: limbs >= cap not all ones, bit2: some limb < cap non-zero
+    __shared__ __attribute__((aligned(16))) int8_t LA[RECON_ROWS * LDS_PITCH];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, h = lane >> 5, c32 = lane & 31;
+    const int rb = (int)(blk % nrb);
+    const int64_t e0 = (blk / nrb) * RECON_ENT;
+    const bool first = rb == 0;
+    const uint32_t *yp = Y + e0 + 64 * wave + c32;
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t e0 = blk * 64;
-    const int64_t lim = min(cap, out_limbs);
-
-    // phase 0: beta (rounded fixed-point sum of y_q/p_q) and the count of negative digits
-    {
-        int el = tid & 63, ph = tid >> 6;
-        int64_t fix = 0;
-        int32_t cnt = 0;
-        if (e0 + el < nent)
-            for (int q = ph; q < P; q += 4) {
-                int32_t yc = unslice4(Y[(int64_t)q * ystride + e0 + el]);
-                uint32_t p = pc[q].p;
-                uint32_t y = yc < 0 ? (uint32_t)(yc + (int32_t)p) : (uint32_t)yc;
-                fix += (int64_t)(((uint64_t)y * pc[q].fx) >> 31);
-                cnt += yc < 0;
-            }
-        s_fix[ph][el] = fix;
-        s_cnt[ph][el] = cnt;
-        __syncthreads();
-        if (tid < 64) {
-            int64_t f = s_fix[0][tid] + s_fix[1][tid] + s_fix[2][tid] + s_fix[3][tid];
-            int32_t n = s_cnt[0][tid] + s_cnt[1][tid] + s_cnt[2][tid] + s_cnt[3][tid];
-            s_gamma[tid] = (int32_t)((f + (1ll << 31)) >> 32) - n;
-            s_carry[tid] = 0;
-            s_top[tid] = -1;
-            s_topn[tid] = -1;
-            s_flags[tid] = 0;
-        }
+    v16i acc[4][2];
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+        for (int ct = 0; ct < 2; ct++) acc[rt][ct] = (v16i){0};
+    // two k-steps of prefetch: HBM latency spans about two k-steps of MFMA work
+    v4i pa[2], pa1[2], yb[2][2], y1[2][2], y2[2][2];
+    uint64_t gfix[2] = {0, 0};
+    int gcnt[2] = {0, 0};
+    // A: 128 rows x 64 bytes of Mc, two 16-byte chunks per thread
+#define ALOAD(dst, k0_)                                                                                \
+    _Pragma("unroll") for (int u = 0; u < 2; u++) {                                                    \
+        const int ch = tid + 256 * u;                                                                  \
+        dst[u] = *(const v4i *)(Mc + (int64_t)(rb * RECON_ROWS + (ch >> 2)) * Kr + (k0_) + 16 * (ch & 3)); \
     }
-
-    const int el_c = tid >> 2, seg = tid & 3;
-    const int64_t e_c = e0 + el_c;
-    uint64_t *cp = nullptr;
-    if (e_c < nent) cp = C + ((e_c / colsC) * Crs + (e_c % colsC)) * W;
-
-    for (int rb = 0; rb < nrb; rb++) {
-        v16i acc[2] = {(v16i){0}, (v16i){0}};
-        for (int k0 = 0; k0 < Kr; k0 += 64) {
-            __syncthreads();
-            for (int u = 0; u < 2; u++) {
-                int ch = tid + 256 * u, row = ch >> 2, prt = ch & 3;
-                *(v4i *)(LA + row * LDS_PITCH + 16 * prt) =
-                    *(const v4i *)(Mc + (int64_t)(rb * 128 + row) * Kr + k0 + 16 * prt);
-            }
-            {
-                int qq = tid >> 4, ec = tid & 15, q = k0 / 4 + qq;
-                uint32_t w4[4] = {0, 0, 0, 0};
-                if (q < P) {
-                    const uint32_t *src = Y + (int64_t)q * ystride + e0 + 4 * ec;
-                    v4i t4 = *(const v4i *)src;
-                    w4[0] = t4[0]; w4[1] = t4[1]; w4[2] = t4[2]; w4[3] = t4[3];
-                }
-                for (int u = 0; u < 4; u++) *(uint32_t *)(LB + (4 * ec + u) * LDS_PITCH + 4 * qq) = w4[u];
-            }
-            __syncthreads();
+    // B fragments straight from Y: K index 4q+s of an entry is byte s of Y[q][e].
+    // Columns 4q+s of Mc are zero for q >= P, so those planes are read
+    // clamped to the last prime (in bounds, multiplied by zero).
+#define YLOAD(dst, k0_)                                                                  \
+    _Pragma("unroll") for (int ct = 0; ct < 2; ct++)                                     \
+    _Pragma("unroll") for (int kk = 0; kk < 2; kk++)                                     \
+    _Pragma("unroll") for (int i = 0; i < 4; i++) {                                      \
+        const int q = min((k0_) / 4 + 8 * kk + 4 * h + i, P - 1);                        \
+        dst[ct][kk][i] = (int)yp[(int64_t)q * ystride + 32 * ct];                        \
+    }
+    ALOAD(pa, 0);
+    YLOAD(yb, 0);
+    if (64 < Kr) {
+        ALOAD(pa1, 64);
+        YLOAD(y1, 64);
+    }
+    for (int k0 = 0; k0 < Kr; k0 += 64) {
+        __syncthreads();
 #pragma unroll
-            for (int kk = 0; kk < 2; kk++) {
-                const int ko = kk * 32 + 16 * (lane >> 5);
-                v4i af = *(const v4i *)(LA + (wave * 32 + (lane & 31)) * LDS_PITCH + ko);
-#pragma unroll
-                for (int ct = 0; ct < 2; ct++) {
-                    v4i bf = *(const v4i *)(LB + (ct * 32 + (lane & 31)) * LDS_PITCH + ko);
-                    acc[ct] = mfma32(af, bf, acc[ct]);
-                }
-            }
+        for (int u = 0; u < 2; u++) {
+            const int ch = tid + 256 * u;
+            *(v4i *)(LA + (ch >> 2) * LDS_PITCH + 16 * (ch & 3)) = pa[u];
         }
         __syncthreads();
+        if (k0 + 64 < Kr) {
+#pragma unroll
+            for (int u = 0; u < 2; u++) pa[u] = pa1[u];
+        }
+        if (k0 + 128 < Kr) {
+            ALOAD(pa1, k0 + 128);
+            YLOAD(y2, k0 + 128);
+        }
+        if (first) {
+#pragma unroll
+            for (int kk = 0; kk < 2; kk++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int q = k0 / 4 + 8 * kk + 4 * h + i;
+                    if (q < P) {
+                        const uint32_t p = pc[q].p;
+                        const uint64_t fx = pc[q].fx;
+#pragma unroll
+                        for (int ct = 0; ct < 2; ct++) {
+                            const int32_t yc = unslice4((uint32_t)yb[ct][kk][i]);
+                            const uint32_t y = yc < 0 ? (uint32_t)(yc + (int32_t)p) : (uint32_t)yc;
+                            gfix[ct] += ((uint64_t)y * fx) >> 31;
+                            gcnt[ct] += yc < 0;
+                        }
+                    }
+                }
+        }
+#pragma unroll
+        for (int kk = 0; kk < 2; kk++)
+#pragma unroll
+            for (int rt = 0; rt < 4; rt++) {
+                const v4i af = *(const v4i *)(LA + (rt * 32 + c32) * LDS_PITCH + kk * 32 + 16 * h);
+#pragma unroll
+                for (int ct = 0; ct < 2; ct++) acc[rt][ct] = mfma32(af, yb[ct][kk], acc[rt][ct]);
+            }
 #pragma unroll
         for (int ct = 0; ct < 2; ct++)
 #pragma unroll
-            for (int r = 0; r < 16; r++)
-                LS[ct * 32 + (lane & 31)][wave * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)] = acc[ct][r];
-        __syncthreads();
-
-        // carry phase: thread (entry el_c, segment seg) owns words seg*8 .. seg*8+7 of the block
-        uint32_t dg[8];
-        int64_t carry = 0;
-        const int64_t gam = s_gamma[el_c];
+            for (int kk = 0; kk < 2; kk++) {
+                yb[ct][kk] = y1[ct][kk];
+                y1[ct][kk] = y2[ct][kk];
+            }
+    }
+#undef ALOAD
+#undef YLOAD
+    // row 8g + 4h + u of row tile rt is byte u of word rb*32 + rt*8 + 2g + h
 #pragma unroll
-        for (int vv = 0; vv < 8; vv++) {
-            int wb = seg * 8 + vv;
-            int64_t x = (int64_t)LS[el_c][4 * wb] + ((int64_t)LS[el_c][4 * wb + 1] << 8) +
-                        ((int64_t)LS[el_c][4 * wb + 2] << 16) + ((int64_t)LS[el_c][4 * wb + 3] << 24);
-            x -= gam * (int64_t)M32[rb * 32 + wb];
-            x += carry;
-            dg[vv] = (uint32_t)x;
-            carry = x >> 32;
-        }
-        bool all1 = true, all0 = true;
+    for (int rt = 0; rt < 4; rt++)
 #pragma unroll
-        for (int vv = 1; vv < 8; vv++) { all1 &= dg[vv] == 0xffffffffu; all0 &= dg[vv] == 0; }
-        s_seg[el_c][seg][0] = carry;
-        s_seg[el_c][seg][1] = dg[0];
-        s_segf[el_c][seg] = (all1 ? 1 : 0) | (all0 ? 2 : 0);
-        __syncthreads();
-        if (seg == 0) {
-            int64_t cin = s_carry[el_c];
+        for (int ct = 0; ct < 2; ct++) {
+            const int64_t e = e0 + 64 * wave + 32 * ct + c32;
+#pragma unroll
             for (int g = 0; g < 4; g++) {
-                s_cin[el_c][g] = cin;
-                int64_t t0 = s_seg[el_c][g][1] + cin;
-                int64_t ov = 0;
-                if (t0 >= (1ll << 32) && (s_segf[el_c][g] & 1)) ov = 1;
-                if (t0 < 0 && (s_segf[el_c][g] & 2)) ov = -1;
-                cin = s_seg[el_c][g][0] + ov;
+                const int t = rb * (RECON_ROWS / 4) + rt * 8 + 2 * g + h;
+                const int64_t v = (int64_t)acc[rt][ct][4 * g] + ((int64_t)acc[rt][ct][4 * g + 1] << 8) +
+                                  ((int64_t)acc[rt][ct][4 * g + 2] << 16) + ((int64_t)acc[rt][ct][4 * g + 3] << 24);
+                if (t < nw && e < nent) Wv[(int64_t)t * ystride + e] = v;
             }
-            s_carry[el_c] = cin;
         }
-        __syncthreads();
-        int64_t cin = s_cin[el_c][seg];
+    if (first) {
 #pragma unroll
-        for (int vv = 0; vv < 8; vv++) {
-            int64_t x = (int64_t)dg[vv] + cin;
-            dg[vv] = (uint32_t)x;
-            cin = x >> 32;
+        for (int ct = 0; ct < 2; ct++) {
+            const uint32_t lo = __shfl_xor((unsigned)gfix[ct], 32), hi = __shfl_xor((unsigned)(gfix[ct] >> 32), 32);
+            const uint64_t f = gfix[ct] + ((uint64_t)hi << 32 | lo);
+            const int cnt = gcnt[ct] + __shfl_xor(gcnt[ct], 32);
+            const int64_t e = e0 + 64 * wave + 32 * ct + c32;
+            if (h == 0 && e < nent) gamma[e] = (int32_t)((f + (1ull << 31)) >> 32) - cnt;
         }
-        if (cp) {
-            int topl = -1, fl = 0;
-#pragma unroll
-            for (int m = 0; m < 4; m++) {
-                int64_t L = (int64_t)rb * 16 + seg * 4 + m;
-                uint64_t limb = (uint64_t)dg[2 * m] | ((uint64_t)dg[2 * m + 1] << 32);
-                if (L < lim) {
-                    cp[L] = limb;
-                    if (limb) { topl = (int)L; fl |= 4; }
-                } else if (L < out_limbs) {
-                    if (limb) fl |= 1;
-                    if (limb != ~0ull) fl |= 2;
-                }
-            }
-            if (topl >= 0) atomicMax(&s_top[el_c], topl);
-            if (fl) atomicOr(&s_flags[el_c], fl);
-        }
-    }
-    __threadfence();
-    __syncthreads();
-
-    // negative results: |C| = ~w + 1 over the stored limbs; the limbs are
-    // split in four contiguous ranges and the +1 passes a range iff all of
-    // its limbs are zero.  No early exits: every thread reaches each barrier.
-    const bool neg = cp && s_carry[el_c] < 0;
-    const int64_t per = (lim + 3) / 4, lo = seg * per, hi = min(lim, lo + per);
-    bool allz = true;
-    if (neg)
-        for (int64_t L = lo; L < hi; L++) allz &= cp[L] == 0;
-    s_segf[el_c][seg] = allz;
-    __syncthreads();
-    if (neg) {
-        int cin = 1;
-        for (int g = 0; g < seg; g++) cin &= s_segf[el_c][g];
-        int topl = -1;
-        for (int64_t L = lo; L < hi; L++) {
-            uint64_t m = ~cp[L] + (uint64_t)cin;
-            cin = cin && m == 0;
-            cp[L] = m;
-            if (m) topl = (int)L;
-        }
-        if (topl >= 0) atomicMax(&s_topn[el_c], topl);
-    }
-    __syncthreads();
-    if (cp && seg == 0) {
-        const int fl = s_flags[el_c];
-        bool bad = false;
-        if (out_limbs > cap) bad = neg ? ((fl & 2) || !(fl & 4)) : (fl & 1);
-        if (bad) atomicOr(err, DEV_ERR_CAPACITY);
-        int64_t sz = (neg ? s_topn[el_c] : s_top[el_c]) + 1;
-        cp[W - 1] = (uint64_t)(neg ? -sz : sz);
     }
 }
+
+constexpr int CARRY_ENT = 256;   // entries per workgroup
+constexpr int CARRY_PITCH = 17;  // limbs per LDS row (16 + 1 padding)
+
+__device__ __forceinline__ void recon_carry_block(int64_t blk, const int64_t *__restrict__ Wv, int64_t ystride,
+    int nw, const int32_t *__restrict__ gamma, const uint32_t *__restrict__ M32, int64_t nent, int64_t colsC,
+    uint64_t *__restrict__ C, int64_t Crs, int64_t W, int64_t cap, unsigned *__restrict__ err)
+{
+    __shared__ uint64_t LS[CARRY_ENT * CARRY_PITCH];
+    __shared__ int64_t s_base[CARRY_ENT];
+    const int tid = threadIdx.x;
+    const int64_t e = blk * CARRY_ENT + tid;
+    const bool valid = e < nent;
+    s_base[tid] = valid ? ((e / colsC) * Crs + e % colsC) * W : -1;
+    const int64_t gam = valid ? gamma[e] : 0;
+    const int64_t *wp = Wv + (valid ? e : 0);
+
+    // Sign: x_t = Wv[t] - gam M32[t] is below 2^56 in magnitude (int32 column
+    // sums of at most 2^17 digit products), so the words below t add less
+    // than 2^25 units of 2^(32t); once the top part H_t reaches 2^28 in
+    // magnitude its sign is the sign of the whole, and below that bound
+    // H_t * 2^32 + x_t cannot overflow.
+    bool neg = false;
+    if (valid) {
+        int64_t H = 0;
+        for (int t = nw - 1; t >= 0; t--) {
+            H = H * 4294967296ll + (wp[(int64_t)t * ystride] - gam * (int64_t)M32[t]);
+            if (H >= (1ll << 28) || H <= -(1ll << 28)) break;
+        }
+        neg = H < 0;
+    }
+
+    const int64_t nlimb = nw / 2, lim = min(cap, nlimb);
+    int64_t carry = 0;
+    int64_t top = -1;
+    bool hi_nz = false;
+    for (int64_t c0 = 0; c0 < nlimb; c0 += 16) {
+        int64_t xw[32];
+#pragma unroll
+        for (int j = 0; j < 32; j++) {
+            const int64_t t = 2 * c0 + j;
+            xw[j] = (valid && t < nw) ? wp[t * ystride] - gam * (int64_t)M32[t] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            int64_t x0 = neg ? -xw[2 * j] : xw[2 * j];
+            x0 += carry;
+            carry = x0 >> 32;
+            int64_t x1 = neg ? -xw[2 * j + 1] : xw[2 * j + 1];
+            x1 += carry;
+            carry = x1 >> 32;
+            const uint64_t limb = (uint64_t)(uint32_t)x0 | ((uint64_t)(uint32_t)x1 << 32);
+            const int64_t L = c0 + j;
+            if (L < lim) {
+                LS[tid * CARRY_PITCH + j] = limb;
+                if (limb) top = L;
+            } else if (L < nlimb) {
+                hi_nz |= limb != 0;
+            }
+        }
+        __syncthreads();
+        const int64_t n = min<int64_t>(16, lim - c0);
+        if (n > 0)
+            for (int f = tid; f < CARRY_ENT * 16; f += CARRY_ENT) {
+                const int ent = f >> 4, j = f & 15;
+                const int64_t base = s_base[ent];
+                if (j < n && base >= 0) C[base + c0 + j] = LS[ent * CARRY_PITCH + j];
+            }
+        __syncthreads();
+    }
+    if (valid) {
+        if (hi_nz) atomicOr(err, DEV_ERR_CAPACITY);
+        C[s_base[tid] + W - 1] = (uint64_t)(neg ? -(top + 1) : top + 1);
+    }
+}
+
+// ------------------------------------------------------------------------
+// host side shared by the one-product launches and the batch
+// ------------------------------------------------------------------------
+inline int64_t crt_ystride(int64_t nent) { return round_up(nent < 1 ? 1 : nent, 256); }
+
+// Device pointers and sizes of a plan (mgx_crt_plan is private to crt.hip).
+struct PlanView {
+    const PrimeConst *pc;
+    const int8_t *cst, *mc;
+    const uint32_t *m32;
+    int64_t P, kb, Kr, rows4pad, Lbp, out_limbs, nw;
+};
+int plan_view(const mgx_crt_plan *pl, PlanView *v);
+int cached_plan(int64_t K, int64_t bitsA, int64_t bitsB, const mgx_crt_plan **out);
 
 }  // namespace mgx

@@ -100,6 +100,8 @@ def _declare(L):
     L.mgx_get_param.restype = c_i64
     L.mgx_crt_plan_primes.argtypes = [c_vp]
     L.mgx_crt_plan_primes.restype = c_i64
+    L.mgx_crt_y_stride.argtypes = [c_i64, c_i64]
+    L.mgx_crt_y_stride.restype = c_i64
 
 
 def check(status):

@@ -83,7 +83,7 @@ def main():
     rpad, kpad = rp.value, kp.value
     RA = mgx.DeviceBuffer(4 * P * rpad * kpad)
     RB = mgx.DeviceBuffer(4 * P * rpad * kpad)
-    ystride = (n * n + 63) // 64 * 64
+    ystride = L.mgx_crt_y_stride(n, n)
     Y = mgx.DeviceBuffer(P * ystride * 4)
     ev = [mgx.c_vp() for _ in range(5)]
     for e in ev:

@@ -192,8 +192,11 @@ int mgx_crt_mfma_matmul(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t 
                         void *stream);
 /* Stage kernels, exposed for testing and profiling.  Residues are planar
  * int8 slices R[q][s][row][k] with row pitch kpad and rows padded to rpad
- * (mgx_crt_residue_dims); Y is the per-prime output C mod p_q as
- * [q][K*N] uint32 in [0, p_q). */
+ * (mgx_crt_residue_dims).  Y is [q][ystride] uint32 with ystride =
+ * mgx_crt_y_stride(K, N) (K*N rounded up to 256): entry i*N+j of plane q holds
+ * y_q = (C_ij mod p_q) (M/p_q)^-1 mod p_q, centred and written as four
+ * balanced int8 digits (byte s = digit s).  mgx_crt_reconstruct takes its own
+ * stream-ordered scratch (about 8 * ystride bytes per 32-bit word of M). */
 int mgx_crt_residue_dims(const mgx_crt_plan *plan, int64_t rows, int64_t K, int64_t *rpad, int64_t *kpad);
 int mgx_crt_reduce(const mgx_crt_plan *plan, int transpose, int64_t rows, int64_t K, const mgx_limb *X,
                    int64_t Xrs, int64_t cap, int64_t nbytes, int8_t *R, void *stream);
@@ -201,6 +204,7 @@ int mgx_crt_residue_gemm(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t
                          const int8_t *RB, uint32_t *Y, void *stream);
 int mgx_crt_reconstruct(const mgx_crt_plan *plan, int64_t K, int64_t N, const uint32_t *Y, mgx_limb *C,
                         int64_t Crs, int64_t cap, void *stream);
+int64_t mgx_crt_y_stride(int64_t K, int64_t N);
 
 /* ---------------- kept reference CRT API (mat-gpu/gpu_crt.h:6-14; host) ---------------- */
 /* Same arguments and meaning as gpu_crt_* on entries of capacity cap, using

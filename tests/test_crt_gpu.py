@@ -73,7 +73,8 @@ def test_residue_gemm_kernel(gpu):
         dA, dB = mgx.DeviceBuffer.from_array(a), mgx.DeviceBuffer.from_array(b)
         dRA = mgx.DeviceBuffer(4 * len(ps) * rp.value * kp.value)
         dRB = mgx.DeviceBuffer(4 * len(ps) * rpb.value * kp.value)
-        ystride = (K * N + 63) // 64 * 64
+        ystride = mgx.lib().mgx_crt_y_stride(K, N)
+        assert ystride == (K * N + 255) // 256 * 256
         dY = mgx.DeviceBuffer(len(ps) * ystride * 4)
         mgx.call("mgx_crt_reduce", plan, 0, K, M, dA.ptr, M, cap, 0, dRA.ptr, None)
         mgx.call("mgx_crt_reduce", plan, 1, N, M, dB.ptr, N, cap, 0, dRB.ptr, None)
