@@ -28,7 +28,7 @@ struct GemmJob {
     const PrimeConst *pc;
     const int8_t *RA, *RB;
     int64_t planeA, planeB, kpad, rowsC, colsC, total, ystride;
-    int ntm, ntn;
+    int ntm, ntn, P;
     uint32_t *Y;
 };
 struct ReconJob {
@@ -77,6 +77,7 @@ __global__ void reset_counters_kernel(unsigned long long *c, int n)
     if ((int)threadIdx.x < n) c[threadIdx.x] = 0;
 }
 
+template <int S>
 __global__ __launch_bounds__(256) void batch_reduce_kernel(Queue q, const ReduceJob *jobs, unsigned *err)
 {
     int j;
@@ -84,11 +85,11 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(Queue q, const Reduce
     while (next_item(q, j, t)) {
         const ReduceJob &b = jobs[j];
         if (b.resident)
-            reduce_tile_res(t, b.pc, b.cst, b.P, b.nrb, b.kb, b.X, b.Xrs, b.W, b.transpose, b.rows, b.K, b.kpad, b.R,
-                            b.plane, err);
+            reduce_tile_res<S>(t, b.pc, b.cst, b.P, b.nrb, b.kb, b.X, b.Xrs, b.W, b.transpose, b.rows, b.K, b.kpad,
+                               b.R, b.plane, err);
         else
-            reduce_tile(t, b.pc, b.cst, b.P, b.nrb, b.kb, b.X, b.Xrs, b.W, b.transpose, b.rows, b.K, b.kpad, b.nent,
-                        b.R, b.plane, err);
+            reduce_tile<S>(t, b.pc, b.cst, b.P, b.nrb, b.kb, b.X, b.Xrs, b.W, b.transpose, b.rows, b.K, b.kpad, b.nent,
+                           b.R, b.plane, err);
         __syncthreads();
     }
 }
@@ -106,13 +107,27 @@ __global__ __launch_bounds__(256, 1) void batch_gemm_kernel(Queue q, const GemmJ
     }
 }
 
+template <bool CHUNKED>
+__global__ __launch_bounds__(256, 1) void batch_gemm2_kernel(Queue q, const GemmJob *jobs)
+{
+    int j;
+    int64_t t;
+    while (next_item(q, j, t)) {
+        const GemmJob &g = jobs[j];
+        gemm_tile2<CHUNKED>(t, g.pc, g.P, g.RA, g.RB, g.planeA, g.planeB, g.kpad, g.rowsC, g.colsC, g.ntm, g.ntn,
+                            g.Y, g.ystride);
+        __syncthreads();
+    }
+}
+
+template <int S>
 __global__ __launch_bounds__(256, 2) void batch_recon_gemm_kernel(Queue q, const ReconJob *jobs)
 {
     int j;
     int64_t t;
     while (next_item(q, j, t)) {
         const ReconJob &r = jobs[j];
-        recon_gemm_tile(t, r.pc, r.P, r.Mc, r.Kr, r.nrb, r.nw, r.Y, r.ystride, r.nent, r.Wv, r.gamma);
+        recon_gemm_tile<S>(t, r.pc, r.P, r.Mc, r.Kr, r.nrb, r.nw, r.Y, r.ystride, r.nent, r.Wv, r.gamma);
         __syncthreads();
     }
 }
@@ -213,26 +228,37 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
     }
     if (st != MGX_OK) { delete b; return st; }
 
-    // ---- job tables (sizes first, then one scratch block, then real pointers)
-    std::vector<ReduceJob> rj;
-    std::vector<GemmJob> gj[2];
-    std::vector<ReconJob> cj;
+    // ---- job tables (sizes first, then one scratch block, then real pointers).
+    // Classes: reduce and reconstruct-GEMM by slice count (sc = 1 for S = 2),
+    // residue GEMM by slice count and chunking (2 sc + chunked); the carry
+    // pass takes every CRT product in one queue.
+    std::vector<ReduceJob> rj[2];
+    std::vector<GemmJob> gj[4];
+    std::vector<ReconJob> cj[2], kj;
     std::vector<PlainJob> pj[NCLASS];
-    std::vector<int64_t> rp{0}, gp[2] = {{0}, {0}}, cp{0}, kp{0}, pp[NCLASS];
+    std::vector<int64_t> rp[2], gp[4], cp[2], kp{0}, pp[NCLASS];
+    for (auto &v : rp) v.push_back(0);
+    for (auto &v : gp) v.push_back(0);
+    for (auto &v : cp) v.push_back(0);
     for (auto &v : pp) v.push_back(0);
     struct Dims { int64_t rpadA, rpadB, kpad, ystride; };
     std::vector<Dims> dims(count);
-    std::vector<int> pclass(count, -1), gclass(count, -1);
-    size_t nred = 0, ngem[2] = {0, 0}, nrec = 0, npl[NCLASS] = {0, 0, 0, 0, 0, 0};
+    std::vector<int> pclass(count, -1), crt(count, 0);
+    size_t nred[2] = {0, 0}, ngem[4] = {0, 0, 0, 0}, nrec[2] = {0, 0}, nrec_all = 0;
+    size_t npl[NCLASS] = {0, 0, 0, 0, 0, 0};
     for (int64_t i = 0; i < count; i++) {
         const mgx_product &p = prods[i];
         if (p.K == 0 || p.N == 0) continue;
         if (b->routes[i] == MGX_ROUTE_CRT) {
             Dims &d = dims[i];
+            const int sc = views[i].S == 2;
             d.rpadA = round_up(p.K, 128); d.rpadB = round_up(p.N, 128);
             d.kpad = round_up(p.M, 64); d.ystride = crt_ystride(p.K * p.N);
-            gclass[i] = d.kpad > 8192;
-            nred += 2; ngem[gclass[i]]++; nrec++;
+            crt[i] = 1;
+            nred[sc] += 2;
+            ngem[2 * sc + (d.kpad > (sc ? KCHUNK2 : KCHUNK))]++;
+            nrec[sc]++;
+            nrec_all++;
         } else {
             int64_t words = 2 * ((std::max(bitsA[i], bitsB[i]) + 63) / 64);
             int k = 0;
@@ -244,20 +270,21 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
     Layout L;
     std::vector<int64_t> o_ra(count), o_rb(count), o_y(count), o_wv(count), o_ga(count);
     for (int64_t i = 0; i < count; i++) {
-        if (gclass[i] < 0) continue;
+        if (!crt[i]) continue;
         const Dims &d = dims[i];
-        o_ra[i] = L.take(4 * views[i].P * d.rpadA * d.kpad);
-        o_rb[i] = L.take(4 * views[i].P * d.rpadB * d.kpad);
-        o_y[i] = L.take(views[i].P * d.ystride * 4);
-        o_wv[i] = L.take(views[i].nw * d.ystride * 8);
+        const PlanView &v = views[i];
+        o_ra[i] = L.take(v.S * v.P * d.rpadA * d.kpad);
+        o_rb[i] = L.take(v.S * v.P * d.rpadB * d.kpad);
+        o_y[i] = L.take(ceil_div(v.S * v.P, 4) * d.ystride * 4);
+        o_wv[i] = L.take(v.nw * d.ystride * 8);
         o_ga[i] = L.take(d.ystride * 4);
     }
     const int64_t o_ctr = L.take(64 * 8);
-    const int64_t o_rj = L.take((int64_t)(nred * sizeof(ReduceJob))), o_rp = L.take((int64_t)(nred + 1) * 8);
-    int64_t o_gj[2], o_gp[2], o_pj[NCLASS], o_pp[NCLASS];
-    for (int c = 0; c < 2; c++) { o_gj[c] = L.take((int64_t)(ngem[c] * sizeof(GemmJob))); o_gp[c] = L.take((int64_t)(ngem[c] + 1) * 8); }
-    const int64_t o_cj = L.take((int64_t)(nrec * sizeof(ReconJob))), o_cp = L.take((int64_t)(nrec + 1) * 8);
-    const int64_t o_kp = L.take((int64_t)(nrec + 1) * 8);
+    int64_t o_rj[2], o_rp[2], o_gj[4], o_gp[4], o_cj[2], o_cp[2], o_pj[NCLASS], o_pp[NCLASS];
+    for (int c = 0; c < 2; c++) { o_rj[c] = L.take((int64_t)(nred[c] * sizeof(ReduceJob))); o_rp[c] = L.take((int64_t)(nred[c] + 1) * 8); }
+    for (int c = 0; c < 4; c++) { o_gj[c] = L.take((int64_t)(ngem[c] * sizeof(GemmJob))); o_gp[c] = L.take((int64_t)(ngem[c] + 1) * 8); }
+    for (int c = 0; c < 2; c++) { o_cj[c] = L.take((int64_t)(nrec[c] * sizeof(ReconJob))); o_cp[c] = L.take((int64_t)(nrec[c] + 1) * 8); }
+    const int64_t o_kj = L.take((int64_t)(nrec_all * sizeof(ReconJob))), o_kp = L.take((int64_t)(nrec_all + 1) * 8);
     for (int c = 0; c < NCLASS; c++) { o_pj[c] = L.take((int64_t)(npl[c] * sizeof(PlainJob))); o_pp[c] = L.take((int64_t)(npl[c] + 1) * 8); }
     b->scratch_bytes = L.off;
     if (hipMalloc(&b->scratch, (size_t)b->scratch_bytes) != hipSuccess) {
@@ -268,9 +295,10 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
     char *base = (char *)b->scratch;
     for (int64_t i = 0; i < count; i++) {
         const mgx_product &p = prods[i];
-        if (gclass[i] >= 0) {
+        if (crt[i]) {
             const PlanView &v = views[i];
             const Dims &d = dims[i];
+            const int sc = v.S == 2;
             const int nrb = (int)(v.rows4pad / 64);
             const bool resident = v.kb <= RES_KB;
             for (int t = 0; t < 2; t++) {
@@ -281,23 +309,28 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
                 r.rows = t ? p.N : p.K; r.K = p.M; r.kpad = d.kpad;
                 r.nent = (t ? d.rpadB : d.rpadA) * d.kpad; r.plane = r.nent;
                 r.R = (int8_t *)(base + (t ? o_rb[i] : o_ra[i]));
-                rj.push_back(r);
-                rp.push_back(rp.back() + ceil_div(r.nent, 128) * (resident ? 1 : nrb));
+                rj[sc].push_back(r);
+                rp[sc].push_back(rp[sc].back() + ceil_div(r.nent, 128) * (resident ? 1 : nrb));
             }
             GemmJob g;
             g.pc = v.pc; g.RA = (const int8_t *)(base + o_ra[i]); g.RB = (const int8_t *)(base + o_rb[i]);
             g.planeA = d.rpadA * d.kpad; g.planeB = d.rpadB * d.kpad; g.kpad = d.kpad; g.rowsC = p.K; g.colsC = p.N;
-            g.ntm = (int)(d.rpadA / 128); g.ntn = (int)(d.rpadB / 64); g.total = (int64_t)g.ntm * g.ntn * v.P;
+            g.P = (int)v.P;
+            g.ntm = (int)(d.rpadA / 128);
+            g.ntn = (int)(d.rpadB / (sc ? 128 : 64));
+            g.total = (int64_t)g.ntm * g.ntn * (sc ? ceil_div(v.P, 2) : v.P);
             g.ystride = d.ystride; g.Y = (uint32_t *)(base + o_y[i]);
-            gj[gclass[i]].push_back(g);
-            gp[gclass[i]].push_back(gp[gclass[i]].back() + g.total);
+            const int gc = 2 * sc + (d.kpad > (sc ? KCHUNK2 : KCHUNK));
+            gj[gc].push_back(g);
+            gp[gc].push_back(gp[gc].back() + g.total);
             ReconJob c;
             c.pc = v.pc; c.Mc = v.mc; c.M32 = v.m32; c.Y = g.Y; c.P = (int)v.P; c.Kr = (int)v.Kr;
             c.nrb = (int)ceil_div(4 * v.nw, RECON_ROWS); c.nw = (int)v.nw; c.ystride = d.ystride;
             c.nent = p.K * p.N; c.colsC = p.N; c.Crs = p.Crs; c.W = p.cap + 1; c.cap = p.cap; c.C = (uint64_t *)p.C;
             c.Wv = (int64_t *)(base + o_wv[i]); c.gamma = (int32_t *)(base + o_ga[i]);
-            cj.push_back(c);
-            cp.push_back(cp.back() + ceil_div(c.nent, RECON_ENT) * c.nrb);
+            cj[sc].push_back(c);
+            cp[sc].push_back(cp[sc].back() + ceil_div(c.nent, RECON_ENT) * c.nrb);
+            kj.push_back(c);
             kp.push_back(kp.back() + ceil_div(c.nent, CARRY_ENT));
         } else if (pclass[i] >= 0) {
             PlainJob q;
@@ -311,24 +344,27 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
         if (n) MGX_HIP(hipMemcpy(base + o, src, n, hipMemcpyHostToDevice));
         return MGX_OK;
     };
-    st = up(o_rj, rj.data(), rj.size() * sizeof(ReduceJob));
-    if (st == MGX_OK) st = up(o_rp, rp.data(), rp.size() * 8);
     for (int c = 0; c < 2 && st == MGX_OK; c++) {
+        st = up(o_rj[c], rj[c].data(), rj[c].size() * sizeof(ReduceJob));
+        if (st == MGX_OK) st = up(o_rp[c], rp[c].data(), rp[c].size() * 8);
+        if (st == MGX_OK) st = up(o_cj[c], cj[c].data(), cj[c].size() * sizeof(ReconJob));
+        if (st == MGX_OK) st = up(o_cp[c], cp[c].data(), cp[c].size() * 8);
+    }
+    for (int c = 0; c < 4 && st == MGX_OK; c++) {
         st = up(o_gj[c], gj[c].data(), gj[c].size() * sizeof(GemmJob));
         if (st == MGX_OK) st = up(o_gp[c], gp[c].data(), gp[c].size() * 8);
     }
-    if (st == MGX_OK) st = up(o_cj, cj.data(), cj.size() * sizeof(ReconJob));
-    if (st == MGX_OK) st = up(o_cp, cp.data(), cp.size() * 8);
+    if (st == MGX_OK) st = up(o_kj, kj.data(), kj.size() * sizeof(ReconJob));
     if (st == MGX_OK) st = up(o_kp, kp.data(), kp.size() * 8);
     for (int c = 0; c < NCLASS && st == MGX_OK; c++) {
         st = up(o_pj[c], pj[c].data(), pj[c].size() * sizeof(PlainJob));
         if (st == MGX_OK) st = up(o_pp[c], pp[c].data(), pp[c].size() * 8);
     }
     if (st != MGX_OK) { mgx_batch_destroy(b); return st; }
-    const int64_t off_counters = o_ctr;
 
-    // ---- capture: reset counters, plain classes, reduce, gemm x2, reconstruct
-    unsigned long long *ctr = (unsigned long long *)(base + off_counters);
+    // ---- capture: reset counters, plain classes, reduce, residue GEMM, reconstruct (GEMM, carry).
+    // Counter slots: 0-1 reduce, 2-5 residue GEMM, 6-7 reconstruct GEMM, 8 carry, 9.. plain classes.
+    unsigned long long *ctr = (unsigned long long *)(base + o_ctr);
     unsigned *err = device_error_word();
     const int toom = (int)param(MGX_PARAM_TOOM22);
     auto queue = [&](int slot, int64_t o_prefix, size_t njobs) {
@@ -345,7 +381,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
         hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, cs, ctr, 64);
         for (int c = 0; c < NCLASS; c++) {
             if (pj[c].empty()) continue;
-            Queue q = queue(5 + c, o_pp[c], pj[c].size());
+            Queue q = queue(9 + c, o_pp[c], pj[c].size());
             int grid = (int)std::min<int64_t>(pp[c].back(), 2048);
             const PlainJob *jobs = (const PlainJob *)(base + o_pj[c]);
             switch (NB_CLASSES[c]) {
@@ -357,28 +393,30 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             default: launch_plain<8>(grid, cs, q, jobs, toom, err); break;
             }
         }
-        if (!rj.empty()) {
-            int grid = (int)std::min<int64_t>(rp.back(), 1024);
-            hipLaunchKernelGGL(batch_reduce_kernel, dim3(grid), dim3(256), 0, cs, queue(0, o_rp, rj.size()),
-                               (const ReduceJob *)(base + o_rj), err);
-        }
         for (int c = 0; c < 2; c++) {
+            if (rj[c].empty()) continue;
+            int grid = (int)std::min<int64_t>(rp[c].back(), 1024);
+            hipLaunchKernelGGL(c ? batch_reduce_kernel<2> : batch_reduce_kernel<4>, dim3(grid), dim3(256), 0, cs,
+                               queue(c, o_rp[c], rj[c].size()), (const ReduceJob *)(base + o_rj[c]), err);
+        }
+        for (int c = 0; c < 4; c++) {
             if (gj[c].empty()) continue;
             int grid = (int)std::min<int64_t>(gp[c].back(), 256);
-            if (c == 0)
-                hipLaunchKernelGGL(batch_gemm_kernel<false>, dim3(grid), dim3(256), 0, cs, queue(1, o_gp[0], gj[0].size()),
-                                   (const GemmJob *)(base + o_gj[0]));
-            else
-                hipLaunchKernelGGL(batch_gemm_kernel<true>, dim3(grid), dim3(256), 0, cs, queue(2, o_gp[1], gj[1].size()),
-                                   (const GemmJob *)(base + o_gj[1]));
+            auto kern = c == 0 ? batch_gemm_kernel<false> : c == 1 ? batch_gemm_kernel<true>
+                      : c == 2 ? batch_gemm2_kernel<false> : batch_gemm2_kernel<true>;
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, cs, queue(2 + c, o_gp[c], gj[c].size()),
+                               (const GemmJob *)(base + o_gj[c]));
         }
-        if (!cj.empty()) {
-            int grid = (int)std::min<int64_t>(cp.back(), 1024);
-            hipLaunchKernelGGL(batch_recon_gemm_kernel, dim3(grid), dim3(256), 0, cs, queue(3, o_cp, cj.size()),
-                               (const ReconJob *)(base + o_cj));
-            grid = (int)std::min<int64_t>(kp.back(), 1024);
-            hipLaunchKernelGGL(batch_recon_carry_kernel, dim3(grid), dim3(256), 0, cs, queue(4, o_kp, cj.size()),
-                               (const ReconJob *)(base + o_cj), err);
+        for (int c = 0; c < 2; c++) {
+            if (cj[c].empty()) continue;
+            int grid = (int)std::min<int64_t>(cp[c].back(), 1024);
+            hipLaunchKernelGGL(c ? batch_recon_gemm_kernel<2> : batch_recon_gemm_kernel<4>, dim3(grid), dim3(256), 0,
+                               cs, queue(6 + c, o_cp[c], cj[c].size()), (const ReconJob *)(base + o_cj[c]));
+        }
+        if (!kj.empty()) {
+            int grid = (int)std::min<int64_t>(kp.back(), 1024);
+            hipLaunchKernelGGL(batch_recon_carry_kernel, dim3(grid), dim3(256), 0, cs, queue(8, o_kp, kj.size()),
+                               (const ReconJob *)(base + o_kj), err);
         }
         e = hipStreamEndCapture(cs, &b->graph);
     }

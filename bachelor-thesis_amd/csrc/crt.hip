@@ -1,17 +1,20 @@
 // crt.hip -- the multi-modular route on gfx950 INT8 MFMA.
 //
-// C = A*B for signed big-integer matrices, exactly, through P word-size
-// primes p_q < 2^31 with M = prod p_q > 4 max|C|:
+// C = A*B for signed big-integer matrices, exactly, through P primes with
+// M = prod p_q > 4 max|C|: S = 2 digits per residue with primes below 65280
+// (the default while enough of them exist), else S = 4 with primes below 2^31.
+// Two digits need 4 slice products per prime, four need 16, and the 16-bit
+// primes are only twice as many, so S = 2 halves the residue GEMM:
 //
 //   1. reduce:      every entry of A and B mod p_q, centred to |r| <= (p-1)/2
-//                   and split into four balanced int8 digits (planar slices).
+//                   and split into S balanced int8 digits (planar slices).
 //                   Itself an int8 GEMM: entry bytes (offset by -128) times the
 //                   balanced digits of 2^(8b) mod p_q.
-//   2. residue GEMM: per prime, the 16 slice products A^(s) B^(t) on
-//                   v_mfma_i32_32x32x32_i8, accumulated per diagonal s+t in
-//                   int32 over K-chunks of at most 8192, folded mod p_q in
+//   2. residue GEMM: per prime, the S^2 slice products A^(s) B^(t) on INT8
+//                   MFMA, accumulated per diagonal s+t in int32 over K-chunks
+//                   (8192 for S = 4, 32768 for S = 2), folded mod p_q in
 //                   64-bit integers (Barrett), then y_q = c_q (M/p_q)^-1 mod p_q
-//                   is written as four balanced int8 digits.
+//                   is written as S balanced int8 digits.
 //   3. reconstruct: explicit CRT, C = sum_q y_q (M/p_q) - beta M, again an int8
 //                   GEMM (the digits of (M/p_q) 256^s against the digits of
 //                   y_q) whose epilogue leaves one signed 32-bit-word column
@@ -66,20 +69,22 @@ __global__ __launch_bounds__(256) void scan_bits_kernel(const uint64_t *X, int64
 // ------------------------------------------------------------------------
 // one-product launches: one workgroup per tile
 // ------------------------------------------------------------------------
+template <int S>
 __global__ __launch_bounds__(256) void crt_reduce_kernel(
     const PrimeConst *__restrict__ pc, const int8_t *__restrict__ cst, int P, int nrb, int kb,
     const uint64_t *__restrict__ X, int64_t Xrs, int64_t W, int transpose, int64_t rows, int64_t K,
     int64_t kpad, int64_t nent, int8_t *__restrict__ R, int64_t plane, unsigned *__restrict__ err)
 {
-    reduce_tile(blockIdx.x, pc, cst, P, nrb, kb, X, Xrs, W, transpose, rows, K, kpad, nent, R, plane, err);
+    reduce_tile<S>(blockIdx.x, pc, cst, P, nrb, kb, X, Xrs, W, transpose, rows, K, kpad, nent, R, plane, err);
 }
 
+template <int S>
 __global__ __launch_bounds__(256) void crt_reduce_res_kernel(
     const PrimeConst *__restrict__ pc, const int8_t *__restrict__ cst, int P, int nrb, int kb,
     const uint64_t *__restrict__ X, int64_t Xrs, int64_t W, int transpose, int64_t rows, int64_t K,
     int64_t kpad, int8_t *__restrict__ R, int64_t plane, unsigned *__restrict__ err)
 {
-    reduce_tile_res(blockIdx.x, pc, cst, P, nrb, kb, X, Xrs, W, transpose, rows, K, kpad, R, plane, err);
+    reduce_tile_res<S>(blockIdx.x, pc, cst, P, nrb, kb, X, Xrs, W, transpose, rows, K, kpad, R, plane, err);
 }
 
 template <bool CHUNKED>
@@ -95,6 +100,20 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_kernel(
     gemm_tile<CHUNKED>(v, pc, RA, RB, planeA, planeB, kpad, rowsC, colsC, ntm, ntn, total, Y, ystride);
 }
 
+template <bool CHUNKED>
+__global__ __launch_bounds__(256, 1) void crt_gemm2_kernel(
+    const PrimeConst *__restrict__ pc, int P, const int8_t *__restrict__ RA, const int8_t *__restrict__ RB,
+    int64_t planeA, int64_t planeB, int64_t kpad, int64_t rowsC, int64_t colsC, int ntm, int ntn, int64_t total,
+    uint32_t *__restrict__ Y, int64_t ystride)
+{
+    // XCD-aware order: consecutive tiles (same prime pair) land on one XCD
+    int64_t per = (total + 7) / 8;
+    int64_t v = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (v >= total) return;
+    gemm_tile2<CHUNKED>(v, pc, P, RA, RB, planeA, planeB, kpad, rowsC, colsC, ntm, ntn, Y, ystride);
+}
+
+template <int S>
 __global__ __launch_bounds__(256, 2) void crt_recon_gemm_kernel(
     const PrimeConst *__restrict__ pc, int P, const int8_t *__restrict__ Mc, int Kr, int nrb, int nw,
     const uint32_t *__restrict__ Y, int64_t ystride, int64_t nent, int64_t *__restrict__ Wv,
@@ -104,7 +123,7 @@ __global__ __launch_bounds__(256, 2) void crt_recon_gemm_kernel(
     int64_t per = (total + 7) / 8;
     int64_t v = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
     if (v >= total) return;
-    recon_gemm_tile(v, pc, P, Mc, Kr, nrb, nw, Y, ystride, nent, Wv, gamma);
+    recon_gemm_tile<S>(v, pc, P, Mc, Kr, nrb, nw, Y, ystride, nent, Wv, gamma);
 }
 
 __global__ __launch_bounds__(256) void crt_recon_carry_kernel(
@@ -124,7 +143,8 @@ using namespace mgx;
 
 struct mgx_crt_plan {
     int64_t K, bitsA, bitsB, bound;   // |C| < 2^bound
-    int64_t P, kb, Kr, rows4pad, Lbp, out_limbs;
+    int64_t S;    // balanced int8 digits per residue (2 or 4)
+    int64_t P, kb, Kr, rows4pad, Lbp, out_limbs;   // rows4pad: S*P rounded up to 64
     int64_t nw;   // 32-bit words of the reconstruction's column sums kept (even)
     std::vector<uint32_t> primes;
     PrimeConst *d_pc = nullptr;
@@ -162,19 +182,25 @@ bool is_prime(uint64_t n)
     return true;
 }
 
-std::vector<uint32_t> &prime_cache()
+// Largest admissible prime for S digits: the centred residue must be a
+// sum of S balanced digits, |r| <= 127 * (256^S - 1) / 255.
+uint64_t prime_limit(int S) { return S == 2 ? 65279 : (1ull << 31) - 1; }
+constexpr uint64_t PRIME_FLOOR_S2 = 16384;   // below this, 16-bit primes add too few bits
+
+std::vector<uint32_t> &prime_cache(int S)
 {
-    static std::vector<uint32_t> v;
-    return v;
+    static std::vector<uint32_t> v2, v4;
+    return S == 2 ? v2 : v4;
 }
 
-// primes below 2^31, descending, until their product exceeds 2^bits
-std::vector<uint32_t> primes_for_bits(int64_t bits, hb::Nat &M)
+// primes below the limit for S, descending, until their product exceeds
+// 2^bits; empty when the primes above the floor do not suffice (S = 2)
+std::vector<uint32_t> primes_for_bits(int64_t bits, hb::Nat &M, int S)
 {
-    auto &cache = prime_cache();
+    auto &cache = prime_cache(S);
     std::vector<uint32_t> out;
     M = hb::from_u64(1);
-    uint64_t c = cache.empty() ? (1ull << 31) - 1 : cache.back() - 1;
+    uint64_t c = cache.empty() ? prime_limit(S) : cache.back() - 1;
     size_t i = 0;
     for (;;) {
         // M > 2^bits ?
@@ -189,7 +215,8 @@ std::vector<uint32_t> primes_for_bits(int64_t bits, hb::Nat &M)
         uint32_t p;
         if (i < cache.size()) p = cache[i];
         else {
-            while (!is_prime(c)) c--;
+            while (c >= 3 && !is_prime(c)) c--;
+            if (S == 2 && c < PRIME_FLOOR_S2) { out.clear(); return out; }
             p = (uint32_t)c--;
             cache.push_back(p);
         }
@@ -250,12 +277,27 @@ int mgx_crt_plan_create(mgx_crt_plan **out, int64_t K, int64_t bitsA, int64_t bi
     pl->bitsB = bitsB;
     pl->bound = bitsA + bitsB + clog2(K);
     hb::Nat M;
-    pl->primes = primes_for_bits(pl->bound + 2, M);
-    const int64_t P = (int64_t)pl->primes.size();
+    const int64_t want = param(MGX_PARAM_CRT_SLICES);
+    if (want != 0 && want != 2 && want != 4) {
+        delete pl;
+        return fail(MGX_ERR_ARG, "slice count %lld is not 0, 2 or 4", (long long)want);
+    }
+    pl->S = 4;
+    if (want != 4) {
+        pl->primes = primes_for_bits(pl->bound + 2, M, 2);
+        if (!pl->primes.empty()) pl->S = 2;
+        else if (want == 2) {
+            delete pl;
+            return fail(MGX_ERR_UNSUPPORTED, "a %lld-bit bound needs more primes below 65280 than exist",
+                        (long long)pl->bound);
+        }
+    }
+    if (pl->S == 4) pl->primes = primes_for_bits(pl->bound + 2, M, 4);
+    const int64_t P = (int64_t)pl->primes.size(), S = pl->S;
     pl->P = P;
     pl->kb = round_up(ceil_div(std::max(bitsA, bitsB), 8), 64);
-    pl->rows4pad = round_up(4 * P, 64);
-    pl->Kr = round_up(4 * P, 64);
+    pl->rows4pad = round_up(S * P, 64);
+    pl->Kr = round_up(S * P, 64);
     if (pl->Kr > 131072) {   // int32 column sums of 4P digit products must stay below 2^31
         delete pl;
         return fail(MGX_ERR_UNSUPPORTED, "%lld primes exceed the 32768-prime limit", (long long)P);
@@ -289,9 +331,9 @@ int mgx_crt_plan_create(mgx_crt_plan **out, int64_t K, int64_t bitsA, int64_t bi
             // reduce-GEMM constants: digits of centred 2^(8b) mod p
             int32_t cen = pw > p / 2 ? (int32_t)pw - (int32_t)p : (int32_t)pw;
             int32_t vv = cen;
-            for (int s = 0; s < 4; s++) {
+            for (int s = 0; s < S; s++) {
                 int32_t t = ((vv + 128) & 255) - 128;
-                cst[(size_t)(4 * q + s) * pl->kb + b] = (int8_t)t;
+                cst[(size_t)(S * q + s) * pl->kb + b] = (int8_t)t;
                 vv = (vv - t) >> 8;
             }
             ks = (ks + pw) % p;
@@ -299,9 +341,9 @@ int mgx_crt_plan_create(mgx_crt_plan **out, int64_t K, int64_t bitsA, int64_t bi
         }
         c.kofs = (uint32_t)(ks * 128 % p);
         // reconstruct constants: digits of (M/p) 256^s
-        for (int s = 0; s < 4; s++) {
+        for (int s = 0; s < S; s++) {
             hb::Nat v = hb::shl(Mq, 8 * s);
-            balanced_digits(v, pl->Lbp, mc.data() + 4 * q + s, pl->Kr);
+            balanced_digits(v, pl->Lbp, mc.data() + S * q + s, pl->Kr);
         }
     }
     std::vector<uint32_t> m32(pl->Lbp / 4, 0);
@@ -334,6 +376,7 @@ int mgx_crt_plan_destroy(mgx_crt_plan *pl)
 }
 
 int64_t mgx_crt_plan_primes(const mgx_crt_plan *pl) { return pl ? pl->P : -1; }
+int64_t mgx_crt_plan_slices(const mgx_crt_plan *pl) { return pl ? pl->S : -1; }
 
 int mgx_crt_plan_prime_list(const mgx_crt_plan *pl, uint32_t *out)
 {
@@ -363,14 +406,16 @@ int mgx_crt_reduce(const mgx_crt_plan *pl, int transpose, int64_t rows, int64_t 
     const bool resident = pl->kb <= RES_KB;   // the entries' bytes fit LDS: read them once
     int64_t nblk = ceil_div(nent, 128) * (resident ? 1 : nrb);
     if (nblk > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "matrix too large for one reduce launch");
+    auto res = pl->S == 2 ? crt_reduce_res_kernel<2> : crt_reduce_res_kernel<4>;
+    auto gen = pl->S == 2 ? crt_reduce_kernel<2> : crt_reduce_kernel<4>;
     if (resident)
-        hipLaunchKernelGGL(crt_reduce_res_kernel, dim3((unsigned)nblk), dim3(256), 0, as_stream(stream), pl->d_pc,
-                           pl->d_cst, (int)pl->P, nrb, (int)pl->kb, (const uint64_t *)X, Xrs, cap + 1, transpose,
-                           rows, K, kpad, R, plane, device_error_word());
+        hipLaunchKernelGGL(res, dim3((unsigned)nblk), dim3(256), 0, as_stream(stream), pl->d_pc, pl->d_cst,
+                           (int)pl->P, nrb, (int)pl->kb, (const uint64_t *)X, Xrs, cap + 1, transpose, rows, K, kpad,
+                           R, plane, device_error_word());
     else
-        hipLaunchKernelGGL(crt_reduce_kernel, dim3((unsigned)nblk), dim3(256), 0, as_stream(stream), pl->d_pc,
-                           pl->d_cst, (int)pl->P, nrb, (int)pl->kb, (const uint64_t *)X, Xrs, cap + 1, transpose,
-                           rows, K, kpad, nent, R, plane, device_error_word());
+        hipLaunchKernelGGL(gen, dim3((unsigned)nblk), dim3(256), 0, as_stream(stream), pl->d_pc, pl->d_cst,
+                           (int)pl->P, nrb, (int)pl->kb, (const uint64_t *)X, Xrs, cap + 1, transpose, rows, K, kpad,
+                           nent, R, plane, device_error_word());
     MGX_HIP(hipGetLastError());
     return MGX_OK;
 }
@@ -382,10 +427,27 @@ int mgx_crt_residue_gemm(const mgx_crt_plan *pl, int64_t K, int64_t M, int64_t N
     int64_t rpadA, rpadB, kpad;
     mgx_crt_residue_dims(pl, K, M, &rpadA, &kpad);
     mgx_crt_residue_dims(pl, N, M, &rpadB, nullptr);
+    int64_t ystride = crt_ystride(K * N);
+    if (pl->S == 2) {   // 128 x 128 tiles of prime pairs
+        int ntm = (int)(rpadA / 128), ntn = (int)(rpadB / 128);
+        int64_t total = (int64_t)ntm * ntn * ceil_div(pl->P, 2);
+        int64_t grid = round_up(total, 8);
+        if (grid > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "product too large for one GEMM launch");
+        if (kpad <= KCHUNK2)
+            hipLaunchKernelGGL(crt_gemm2_kernel<false>, dim3((unsigned)grid), dim3(256), 0, as_stream(stream),
+                               pl->d_pc, (int)pl->P, RA, RB, rpadA * kpad, rpadB * kpad, kpad, K, N, ntm, ntn, total,
+                               Y, ystride);
+        else
+            hipLaunchKernelGGL(crt_gemm2_kernel<true>, dim3((unsigned)grid), dim3(256), 0, as_stream(stream),
+                               pl->d_pc, (int)pl->P, RA, RB, rpadA * kpad, rpadB * kpad, kpad, K, N, ntm, ntn, total,
+                               Y, ystride);
+        MGX_HIP(hipGetLastError());
+        return MGX_OK;
+    }
     int ntm = (int)(rpadA / 128), ntn = (int)(rpadB / 64);
     int64_t total = (int64_t)ntm * ntn * pl->P;
     int64_t grid = round_up(total, 8);
-    int64_t ystride = crt_ystride(K * N);
+    if (grid > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "product too large for one GEMM launch");
     // products with K beyond one chunk take the variant that folds between chunks
     if (kpad <= KCHUNK)
         hipLaunchKernelGGL(crt_gemm_kernel<false>, dim3((unsigned)grid), dim3(256), 0, as_stream(stream), pl->d_pc,
@@ -409,8 +471,9 @@ int reconstruct_with(const mgx_crt_plan *pl, int64_t K, int64_t N, const uint32_
     const int nrb = (int)ceil_div(4 * pl->nw, RECON_ROWS);
     const int64_t total = ceil_div(nent, RECON_ENT) * nrb;
     if (round_up(total, 8) > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "product too large for one reconstruct launch");
-    hipLaunchKernelGGL(crt_recon_gemm_kernel, dim3((unsigned)round_up(total, 8)), dim3(256), 0, s, pl->d_pc,
-                       (int)pl->P, pl->d_mc, (int)pl->Kr, nrb, (int)pl->nw, Y, ystride, nent, Wv, gamma, total);
+    hipLaunchKernelGGL(pl->S == 2 ? crt_recon_gemm_kernel<2> : crt_recon_gemm_kernel<4>,
+                       dim3((unsigned)round_up(total, 8)), dim3(256), 0, s, pl->d_pc, (int)pl->P, pl->d_mc,
+                       (int)pl->Kr, nrb, (int)pl->nw, Y, ystride, nent, Wv, gamma, total);
     MGX_HIP(hipGetLastError());
     hipLaunchKernelGGL(crt_recon_carry_kernel, dim3((unsigned)ceil_div(nent, CARRY_ENT)), dim3(256), 0, s, Wv,
                        ystride, (int)pl->nw, gamma, pl->d_m32, nent, N, (uint64_t *)C, Crs, cap + 1, cap,
@@ -505,8 +568,8 @@ int mgx_crt_mfma_matmul(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t 
     mgx_crt_residue_dims(pl, N, M, &rpadB, nullptr);
     // one scratch block: residues of A and B, Y, then the reconstruction's
     // column sums and gamma (which reuse the residue bytes when those are larger)
-    const size_t bytesA = (size_t)(4 * pl->P * rpadA * kpad), bytesB = (size_t)(4 * pl->P * rpadB * kpad);
-    const size_t bytesY = (size_t)pl->P * crt_ystride(K * N) * 4;
+    const size_t bytesA = (size_t)(pl->S * pl->P * rpadA * kpad), bytesB = (size_t)(pl->S * pl->P * rpadB * kpad);
+    const size_t bytesY = (size_t)ceil_div(pl->S * pl->P, 4) * crt_ystride(K * N) * 4;
     const size_t bytesR = recon_scratch_bytes(pl, K * N);
     const size_t offY = (size_t)round_up((int64_t)(bytesA + bytesB), 256);
     const size_t offR = bytesR <= offY ? 0 : (size_t)round_up((int64_t)(offY + bytesY), 256);
@@ -533,7 +596,7 @@ namespace mgx {
 int plan_view(const mgx_crt_plan *pl, PlanView *v)
 {
     v->pc = pl->d_pc; v->cst = pl->d_cst; v->mc = pl->d_mc; v->m32 = pl->d_m32;
-    v->P = pl->P; v->kb = pl->kb; v->Kr = pl->Kr; v->rows4pad = pl->rows4pad; v->Lbp = pl->Lbp;
+    v->S = pl->S; v->P = pl->P; v->kb = pl->kb; v->Kr = pl->Kr; v->rows4pad = pl->rows4pad; v->Lbp = pl->Lbp;
     v->out_limbs = pl->out_limbs;
     v->nw = pl->nw;
     return MGX_OK;
@@ -546,10 +609,10 @@ int plan_view(const mgx_crt_plan *pl, PlanView *v)
 int cached_plan(int64_t K, int64_t bitsA, int64_t bitsB, const mgx_crt_plan **out)
 {
     static std::mutex mu;
-    static std::vector<std::pair<std::array<int64_t, 4>, mgx_crt_plan *>> cache;
+    static std::vector<std::pair<std::array<int64_t, 5>, mgx_crt_plan *>> cache;
     int dev = 0;
     MGX_HIP(hipGetDevice(&dev));
-    std::array<int64_t, 4> key = {dev, clog2(K), bitsA, bitsB};
+    std::array<int64_t, 5> key = {dev, clog2(K), bitsA, bitsB, param(MGX_PARAM_CRT_SLICES)};
     std::lock_guard<std::mutex> lk(mu);
     for (auto &kv : cache)
         if (kv.first == key) {

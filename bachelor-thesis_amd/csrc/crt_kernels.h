@@ -1,7 +1,14 @@
 // crt_kernels.h -- device side of the multi-modular route: per-prime
-// constants, modular helpers and the three tile functions (reduce, residue
-// GEMM, reconstruct) shared by the one-product kernels and the batched
-// persistent kernels.  See crt.hip for the method.
+// constants, modular helpers and the tile functions (reduce, residue GEMM,
+// reconstruct) shared by the one-product kernels and the batched persistent
+// kernels.  See crt.hip for the method.
+//
+// A plan slices every residue into S balanced int8 digits: S = 4 for primes
+// below 2^31, S = 2 for primes below 65280 (|centred residue| <= 32639, the
+// range of two balanced digits).  Rows of the reduce output and columns of
+// the reconstruction constants are indexed S*q + s; the residue GEMM's
+// output Y is packed as 32-bit words of four digits, so word plane w holds
+// prime w (S = 4) or primes 2w and 2w+1 in its low and high halves (S = 2).
 #pragma once
 #include "common.h"
 
@@ -30,6 +37,14 @@ __device__ __forceinline__ uint32_t mod_s64(int64_t x, const PrimeConst &c)
 {
     return mod_u64((uint64_t)(x + (int64_t)c.off), c.p, c.mu);   // |x| < 2^62
 }
+// a * b mod p for a, b < p < 2^16: one 32-bit Barrett step with
+// mu32 = floor(2^32 / p) = mu >> 32 leaves r < 2p
+__device__ __forceinline__ uint32_t mulmod16(uint32_t a, uint32_t b, uint32_t p, uint32_t mu32)
+{
+    const uint32_t u = a * b;
+    const uint32_t r = u - __umulhi(u, mu32) * p;
+    return r >= p ? r - p : r;
+}
 // four balanced base-256 digits of |r| < 2^30, byte s = digit s
 __device__ __forceinline__ uint32_t slice4(int32_t r)
 {
@@ -49,18 +64,37 @@ __device__ __forceinline__ int32_t unslice4(uint32_t w)
            ((int32_t)(int8_t)((w >> 16) & 255) << 16) + ((int32_t)(int8_t)(w >> 24) << 24);
 }
 __device__ __forceinline__ int32_t centre(uint32_t r, uint32_t p) { return r > (p >> 1) ? (int32_t)r - (int32_t)p : (int32_t)r; }
+// two balanced digits of |r| <= 32639 in the low 16 bits
+__device__ __forceinline__ uint32_t slice2(int32_t r)
+{
+    const int32_t t = ((r + 128) & 255) - 128;
+    return (uint32_t)(t & 255) | ((uint32_t)((r - t) >> 8) & 255) << 8;
+}
+__device__ __forceinline__ int32_t unslice2(uint32_t h)
+{
+    return (int32_t)(int8_t)(h & 255) + ((int32_t)(int8_t)((h >> 8) & 255) << 8);
+}
+template <int S>
+__device__ __forceinline__ uint32_t slice_s(int32_t r) { return S == 4 ? slice4(r) : slice2(r); }
 
 __device__ __forceinline__ v16i mfma32(v4i a, v4i b, v16i c)
 {
     return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
+}
+// 16x16x64: lane l gives row (col) l%16, K bytes 16(l/16)..+15; output
+// register r of lane l is row 4(l/16)+r, column l%16
+__device__ __forceinline__ v4i mfma16(v4i a, v4i b, v4i c)
+{
+    return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
 }
 
 constexpr int LDS_PITCH = 80;   // 64 data bytes + 16 padding per row
 
 
 // ------------------------------------------------------------------------
-// 1. reduce.  Output rows (q,s) x entries; tile 64 rows x 128 entries.
+// 1. reduce.  Output rows S*q+s x entries; tile 64 rows x 128 entries.
 // ------------------------------------------------------------------------
+template <int S>
 __device__ __forceinline__ void reduce_tile(int64_t blk,
     const PrimeConst *__restrict__ pc, const int8_t *__restrict__ cst, int P, int nrb, int kb,
     const uint64_t *__restrict__ X, int64_t Xrs, int64_t W, int transpose, int64_t rows, int64_t K,
@@ -119,7 +153,8 @@ __device__ __forceinline__ void reduce_tile(int64_t blk,
         }
     }
 
-    // epilogue: lane owns entry column e, 4 primes x 4 slices per row tile
+    // epilogue: lane owns entry column e; register 4g+u of row tile rt is
+    // row rt*32 + 8g + 4(lane/32) + u = S*q + s of this row block
     const int64_t e = e0 + wave * 32 + (lane & 31);
     if (e >= nent) return;
     int64_t r = e / kpad, k = e % kpad;
@@ -132,22 +167,25 @@ __device__ __forceinline__ void reduce_tile(int64_t blk,
 #pragma unroll
     for (int rt = 0; rt < 2; rt++)
 #pragma unroll
-        for (int g = 0; g < 4; g++) {
-            int q = rb * 16 + rt * 8 + 2 * g + (lane >> 5);
-            if (q >= P) continue;
-            uint32_t word = 0;
-            if (real) {
-                const PrimeConst c = pc[q];
-                int64_t v = (int64_t)acc[rt][4 * g] + ((int64_t)acc[rt][4 * g + 1] << 8) +
-                            ((int64_t)acc[rt][4 * g + 2] << 16) + ((int64_t)acc[rt][4 * g + 3] << 24) + c.kofs;
-                uint32_t res = mod_s64(v, c);
-                if (neg && res) res = c.p - res;
-                word = slice4(centre(res, c.p));
-            }
-            int8_t *o = R + (int64_t)q * 4 * plane + e;
+        for (int g = 0; g < 4; g++)
 #pragma unroll
-            for (int s = 0; s < 4; s++) o[s * plane] = (int8_t)(word >> (8 * s));
-        }
+            for (int w = 0; w < 4 / S; w++) {
+                const int q = rb * (64 / S) + (rt * 32 + 8 * g + 4 * (lane >> 5)) / S + w;
+                if (q >= P) continue;
+                uint32_t word = 0;
+                if (real) {
+                    const PrimeConst c = pc[q];
+                    int64_t v = c.kofs;
+#pragma unroll
+                    for (int s = 0; s < S; s++) v += (int64_t)acc[rt][4 * g + S * w + s] << (8 * s);
+                    uint32_t res = mod_s64(v, c);
+                    if (neg && res) res = c.p - res;
+                    word = slice_s<S>(centre(res, c.p));
+                }
+                int8_t *o = R + (int64_t)q * S * plane + e;
+#pragma unroll
+                for (int s = 0; s < S; s++) o[s * plane] = (int8_t)(word >> (8 * s));
+            }
 }
 
 
@@ -158,6 +196,7 @@ __device__ __forceinline__ void reduce_tile(int64_t blk,
 constexpr int RES_KB = 256;
 constexpr int RES_PITCH = RES_KB + 16;
 
+template <int S>
 __device__ __forceinline__ void reduce_tile_res(int64_t blk,
     const PrimeConst *__restrict__ pc, const int8_t *__restrict__ cst, int P, int nrb, int kb,
     const uint64_t *__restrict__ X, int64_t Xrs, int64_t W, int transpose, int64_t rows, int64_t K,
@@ -168,6 +207,7 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
     __shared__ __attribute__((aligned(16))) int8_t LO[64 * 128];
     __shared__ int64_t s_base[128];   // entry offset in words, -1 for padding
     __shared__ int s_nl[128];         // limbs to read, ~limbs when negative
+    __shared__ uint32_t s_pc[32][4];  // S = 2: p, kofs, floor(2^32/p), a multiple of p in [2^31, 2^31 + p)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t e0 = blk * 128;
     const int cpe = kb / 16;   // 16-byte chunks per entry
@@ -202,14 +242,31 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
         d[1] = v1 ^ 0x8080808080808080ull;
     }
     const int el = wave * 32 + (lane & 31);
+    // the constants of row block rb: 64 rows x kb bytes, at most four 16-byte chunks per thread
+    v4i cpre[4];
+#define CLOAD(rb_)                                                                                    \
+    _Pragma("unroll") for (int u = 0; u < 4; u++) {                                                   \
+        const int c = tid + 256 * u, row = c / cpe, part = c - row * cpe;                             \
+        if (c < 64 * cpe) cpre[u] = *(const v4i *)(cst + (int64_t)((rb_) * 64 + row) * kb + 16 * part); \
+    }
+    CLOAD(0);
     for (int rb = 0; rb < nrb; rb++) {
-        __syncthreads();   // LE staged; LC and LO free again
-        for (int c = tid; c < 64 * cpe; c += 256) {
-            const int row = c / cpe, part = c - row * cpe;
-            *(v4i *)(LC + row * RES_PITCH + 16 * part) =
-                *(const v4i *)(cst + (int64_t)(rb * 64 + row) * kb + 16 * part);
+        __syncthreads();   // LE staged; LC, LO and s_pc free again
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int c = tid + 256 * u, row = c / cpe, part = c - row * cpe;
+            if (c < 64 * cpe) *(v4i *)(LC + row * RES_PITCH + 16 * part) = cpre[u];
+        }
+        if (S == 2 && tid < 32) {
+            const int q = min(rb * 32 + tid, P - 1);
+            const uint32_t p = pc[q].p;
+            s_pc[tid][0] = p;
+            s_pc[tid][1] = pc[q].kofs;
+            s_pc[tid][2] = (uint32_t)(pc[q].mu >> 32);
+            s_pc[tid][3] = ((1u << 31) / p + 1) * p;
         }
         __syncthreads();
+        if (rb + 1 < nrb) { CLOAD(rb + 1); }
         v16i acc[2] = {(v16i){0}, (v16i){0}};
         for (int k32 = 0; k32 < kb / 32; k32++) {
             const int ko = k32 * 32 + 16 * (lane >> 5);
@@ -224,29 +281,42 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
 #pragma unroll
         for (int rt = 0; rt < 2; rt++)
 #pragma unroll
-            for (int g = 0; g < 4; g++) {
-                const int ql = rt * 8 + 2 * g + (lane >> 5), q = rb * 16 + ql;
-                uint32_t word = 0;
-                if (real && q < P) {
-                    const PrimeConst c = pc[q];
-                    const int64_t v = (int64_t)acc[rt][4 * g] + ((int64_t)acc[rt][4 * g + 1] << 8) +
-                                      ((int64_t)acc[rt][4 * g + 2] << 16) + ((int64_t)acc[rt][4 * g + 3] << 24) +
-                                      c.kofs;
-                    uint32_t res = mod_s64(v, c);
-                    if (neg && res) res = c.p - res;
-                    word = slice4(centre(res, c.p));
-                }
+            for (int g = 0; g < 4; g++)
 #pragma unroll
-                for (int s = 0; s < 4; s++) LO[(4 * ql + s) * 128 + el] = (int8_t)(word >> (8 * s));
-            }
+                for (int w = 0; w < 4 / S; w++) {
+                    const int ql = (rt * 32 + 8 * g + 4 * (lane >> 5)) / S + w, q = rb * (64 / S) + ql;
+                    uint32_t word = 0;
+                    if (S == 2 && real && q < P) {
+                        // |acc| <= 2^22 (kb <= 256), so v + off lies in [0, 2^32): 32-bit Barrett
+                        const uint32_t p = s_pc[ql][0], mu32 = s_pc[ql][2];
+                        const uint32_t u = (uint32_t)(acc[rt][4 * g + S * w] + acc[rt][4 * g + S * w + 1] * 256) +
+                                           s_pc[ql][1] + s_pc[ql][3];
+                        uint32_t res = u - __umulhi(u, mu32) * p;   // < 3p
+                        res = res >= p ? res - p : res;
+                        res = res >= p ? res - p : res;
+                        if (neg && res) res = p - res;
+                        word = slice2(centre(res, p));
+                    } else if (S == 4 && real && q < P) {
+                        const PrimeConst c = pc[q];
+                        int64_t v = c.kofs;
+#pragma unroll
+                        for (int s = 0; s < S; s++) v += (int64_t)acc[rt][4 * g + S * w + s] << (8 * s);
+                        uint32_t res = mod_s64(v, c);
+                        if (neg && res) res = c.p - res;
+                        word = slice_s<S>(centre(res, c.p));
+                    }
+#pragma unroll
+                    for (int s = 0; s < S; s++) LO[(S * ql + s) * 128 + el] = (int8_t)(word >> (8 * s));
+                }
         __syncthreads();
         for (int c = tid; c < 64 * 8; c += 256) {
-            const int row = c >> 3, part = c & 7, q = rb * 16 + (row >> 2);
-            if (q < P)
-                *(v4i *)(R + ((int64_t)q * 4 + (row & 3)) * plane + e0 + 16 * part) =
-                    *(const v4i *)(LO + row * 128 + 16 * part);
+            const int row = c >> 3, part = c & 7;
+            const int64_t grow = (int64_t)rb * 64 + row;   // = S*q + s
+            if (grow < (int64_t)S * P)
+                *(v4i *)(R + grow * plane + e0 + 16 * part) = *(const v4i *)(LO + row * 128 + 16 * part);
         }
     }
+#undef CLOAD
 }
 
 // ------------------------------------------------------------------------
@@ -369,6 +439,205 @@ __device__ __forceinline__ void gemm_tile(int64_t v,
 }
 
 // ------------------------------------------------------------------------
+// 2b. residue GEMM for two slices (primes below 65280).  One workgroup
+//     takes a 128 x 128 tile of a pair of primes 2w, 2w+1 in turn; 4 waves
+//     of 64 x 64 on v_mfma_i32_32x32x32_i8, 3 diagonal accumulators
+//     (s + t = 0, 1, 2).  K runs in steps of 64 bytes through a ring of four
+//     LDS stages filled by global_load_lds_dwordx4 (global memory straight
+//     into LDS, no staging registers) three steps ahead, so that an L2 miss
+//     is covered by three steps of MFMA work.  The pair's digits leave as
+//     whole 32-bit words (prime 2w low, 2w+1 high).
+//
+//     Stage layout: [A | B][slice][row 0..127][4 chunks of 16 bytes], the
+//     chunk of K bytes 16c..16c+15 of row r at slot c ^ ((r >> 2) & 3), so
+//     that 8 consecutive rows read at one K chunk hit distinct banks.  One
+//     DMA instruction of a wave writes 1 KB (16 rows) at base + 16*lane;
+//     lane l takes row l/4, slot l%4, i.e. chunk (l & 3) ^ (l >> 4).
+// ------------------------------------------------------------------------
+constexpr int KCHUNK2 = 32768;   // 2 pairs * 32768 * 2^14 = 2^30 < 2^31
+constexpr int G2_STAGES = 4;
+constexpr int G2_OPERAND = 2 * 128 * 64;        // bytes of A (or B) per stage
+constexpr int G2_STAGE = 2 * G2_OPERAND;
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+__device__ __forceinline__ uint32_t lds_offset(const void *p) { return (uint32_t)(uintptr_t)(const lds_void *)p; }
+
+// 16 bytes from g (per lane) to LDS at base + 16 * lane; base is wave-uniform.
+// Issued as inline assembly so that the compiler does not wait on it: the
+// caller orders it with wait_vmcnt and a barrier.
+__device__ __forceinline__ void dma16(const void *g, uint32_t lds_base)
+{
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(lds_base), "v"(g)
+                 : "memory", "m0");
+}
+// wait until at most N vector memory operations of this wave are in flight
+template <int N>
+__device__ __forceinline__ void wait_vmcnt()
+{
+    __builtin_amdgcn_s_waitcnt((N & 15) | (((N >> 4) & 3) << 14) | 0x0F70);
+}
+
+template <bool CHUNKED>
+__device__ __forceinline__ void gemm_tile2(int64_t v,
+    const PrimeConst *__restrict__ pc, int P, const int8_t *__restrict__ RA, const int8_t *__restrict__ RB,
+    int64_t planeA, int64_t planeB, int64_t kpad, int64_t rowsC, int64_t colsC, int ntm, int ntn,
+    uint32_t *__restrict__ Y, int64_t ystride)
+{
+    __shared__ __attribute__((aligned(1024))) int8_t LS[G2_STAGES * G2_STAGE];
+    uint32_t *LW = (uint32_t *)LS;   // after the K loop: the 128 x 128 output words
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wi = wave >> 1, wj = wave & 1, l32 = lane & 31, h = lane >> 5;
+
+    const int64_t tiles = (int64_t)ntm * ntn;
+    const int pair = (int)(v / tiles);
+    const int64_t t = v % tiles;
+    const int64_t i0 = (t / ntn) * 128, j0 = (t % ntn) * 128;
+    uint32_t *Yw = Y + (int64_t)pair * ystride;
+    const int nsteps = (int)(kpad / 64);
+    const uint32_t ls0 = __builtin_amdgcn_readfirstlane(lds_offset(LS));
+    const int dchunk = (lane & 3) ^ (lane >> 4);   // K chunk this lane fetches
+
+    uint32_t hold[2][2][8];   // digits of prime 2w, two 16-bit values per register
+
+    for (int half = 0; half < 2; half++) {
+        const int q = 2 * pair + half;
+        if (q >= P) break;
+        const PrimeConst c = pc[q];
+        const uint32_t mu32 = (uint32_t)(c.mu >> 32);
+        const int8_t *gA = RA + (int64_t)q * 2 * planeA;
+        const int8_t *gB = RB + (int64_t)q * 2 * planeB;
+        // this wave's 8 DMA sources per stage: u -> operand u>>2 ... see ISSUE
+        const int8_t *src[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int idx = wave * 8 + u, op = idx >> 4, s = (idx >> 3) & 1, rb16 = idx & 7;
+            const int64_t row = rb16 * 16 + (lane >> 2);
+            src[u] = op ? gB + s * planeB + (j0 + row) * kpad + 16 * dchunk
+                        : gA + s * planeA + (i0 + row) * kpad + 16 * dchunk;
+        }
+#define ISSUE(k_)                                                                                     \
+        _Pragma("unroll") for (int u = 0; u < 8; u++) {                                               \
+            const int idx = wave * 8 + u, op = idx >> 4, s = (idx >> 3) & 1, rb16 = idx & 7;          \
+            dma16(src[u] + (int64_t)(k_) * 64,                                                         \
+                  __builtin_amdgcn_readfirstlane(ls0 + ((k_) & 3) * G2_STAGE + op * G2_OPERAND +       \
+                                                 (s * 128 + rb16 * 16) * 64));                         \
+        }
+
+        v16i acc[3][2][2];
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+                for (int ct = 0; ct < 2; ct++) acc[d][rt][ct] = (v16i){0};
+
+        // x = sum_d acc_d 2^(8d) mod p (|x| < 2^47), plus the running residue
+        // of earlier chunks; the last fold turns it into y = x (M/p)^-1 mod p,
+        // centred, as two balanced digits
+#define FOLD2(first_, last_)                                                                         \
+        _Pragma("unroll") for (int rt = 0; rt < 2; rt++)                                             \
+        _Pragma("unroll") for (int ct = 0; ct < 2; ct++)                                             \
+        _Pragma("unroll") for (int r = 0; r < 16; r++) {                                             \
+            const int64_t i = i0 + wi * 64 + rt * 32 + 8 * (r >> 2) + 4 * h + (r & 3);               \
+            const int64_t j = j0 + wj * 64 + ct * 32 + l32;                                          \
+            uint16_t *yh = (uint16_t *)(Yw + i * colsC + j) + half;                                  \
+            const bool in = i < rowsC && j < colsC;                                                  \
+            int64_t x = (first_ || !in) ? 0 : (int64_t)*yh;                                          \
+            x += (int64_t)acc[0][rt][ct][r] + ((int64_t)acc[1][rt][ct][r] << 8) +                   \
+                 ((int64_t)acc[2][rt][ct][r] << 16);                                                 \
+            uint32_t res = mod_s64(x, c);                                                            \
+            if (last_) res = slice2(centre(mulmod16(res, c.inv, c.p, mu32), c.p));                   \
+            if (!CHUNKED && last_) {                                                                 \
+                if (half == 0) {                                                                     \
+                    if (r & 1) hold[rt][ct][r >> 1] |= res << 16;                                    \
+                    else hold[rt][ct][r >> 1] = res;                                                 \
+                } else {                                                                             \
+                    LW[(wi * 64 + rt * 32 + 8 * (r >> 2) + 4 * h + (r & 3)) * 128 + wj * 64 + ct * 32 + l32] = \
+                        ((hold[rt][ct][r >> 1] >> (16 * (r & 1))) & 0xffff) | (res << 16);           \
+                }                                                                                    \
+            } else if (in) {                                                                         \
+                *yh = (uint16_t)res;                                                                 \
+            }                                                                                        \
+        }                                                                                            \
+        _Pragma("unroll") for (int d = 0; d < 3; d++)                                                \
+        _Pragma("unroll") for (int rt = 0; rt < 2; rt++)                                             \
+        _Pragma("unroll") for (int ct = 0; ct < 2; ct++) acc[d][rt][ct] = (v16i){0};
+
+        __syncthreads();   // every wave is done with the previous prime's stages
+        ISSUE(0);
+        if (1 < nsteps) { ISSUE(1); }
+        if (2 < nsteps) { ISSUE(2); }
+        int64_t since = 0;
+        bool first = true;
+        for (int k = 0; k < nsteps; k++) {
+            // stage k has landed once only the (up to two) later stages' DMAs are in flight
+            const int later = min(nsteps - 1, k + 2) - k;
+            if (later >= 2) wait_vmcnt<16>();
+            else if (later == 1) wait_vmcnt<8>();
+            else wait_vmcnt<0>();
+            __syncthreads();
+            // stage (k+3)&3 was last read in step k-1, which every wave has finished
+            if (k + 3 < nsteps) { ISSUE(k + 3); }
+            const int8_t *sa = LS + (k & 3) * G2_STAGE, *sb = sa + G2_OPERAND;
+#pragma unroll
+            for (int kk = 0; kk < 2; kk++) {
+                const int ch = 2 * kk + h;
+                v4i af[2][2], bf[2][2];
+#pragma unroll
+                for (int s = 0; s < 2; s++)
+#pragma unroll
+                    for (int x = 0; x < 2; x++) {
+                        const int ra = wi * 64 + x * 32 + l32, rb = wj * 64 + x * 32 + l32;
+                        af[s][x] = *(const v4i *)(sa + (s * 128 + ra) * 64 + 16 * (ch ^ ((ra >> 2) & 3)));
+                        bf[s][x] = *(const v4i *)(sb + (s * 128 + rb) * 64 + 16 * (ch ^ ((rb >> 2) & 3)));
+                    }
+#pragma unroll
+                for (int s = 0; s < 2; s++)
+#pragma unroll
+                    for (int u = 0; u < 2; u++)
+#pragma unroll
+                        for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+                            for (int ct = 0; ct < 2; ct++)
+                                acc[s + u][rt][ct] = mfma32(af[s][rt], bf[u][ct], acc[s + u][rt][ct]);
+            }
+            since += 64;
+            if (CHUNKED && since == KCHUNK2 && k + 1 < nsteps) {
+                FOLD2(first, false);
+                first = false;
+                since = 0;
+            }
+        }
+        __syncthreads();   // the output words reuse the stages
+        if (first) { FOLD2(true, true); }
+        else { FOLD2(false, true); }
+    }
+#undef ISSUE
+#undef FOLD2
+    if (!CHUNKED) {
+        if (2 * pair + 1 >= P) {   // odd prime count: the pair's high half is zero
+#pragma unroll
+            for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+                for (int ct = 0; ct < 2; ct++)
+#pragma unroll
+                    for (int r = 0; r < 16; r++)
+                        LW[(wi * 64 + rt * 32 + 8 * (r >> 2) + 4 * h + (r & 3)) * 128 + wj * 64 + ct * 32 + l32] =
+                            (hold[rt][ct][r >> 1] >> (16 * (r & 1))) & 0xffff;
+        }
+        __syncthreads();
+        // two rows of 128 words per pass, 512 contiguous bytes per row
+        const int64_t j = j0 + (tid & 127);
+        for (int pass = 0; pass < 64; pass++) {
+            const int row = 2 * pass + (tid >> 7);
+            const int64_t i = i0 + row;
+            if (i < rowsC && j < colsC) Yw[i * colsC + j] = LW[row * 128 + (tid & 127)];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
 // 3. reconstruct, in two passes.
 //    (a) recon_gemm_tile: X' = sum_q yc_q (M/p_q), with yc_q the centred
 //        y_q, as an int8 GEMM of the digits of (M/p_q) 256^s (rows = result
@@ -386,6 +655,7 @@ __device__ __forceinline__ void gemm_tile(int64_t v,
 constexpr int RECON_ROWS = 128;   // result bytes per workgroup
 constexpr int RECON_ENT = 256;    // entries per workgroup (64 per wave)
 
+template <int S>
 __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *__restrict__ pc, int P,
     const int8_t *__restrict__ Mc, int Kr, int nrb, int nw, const uint32_t *__restrict__ Y, int64_t ystride,
     int64_t nent, int64_t *__restrict__ Wv, int32_t *__restrict__ gamma)
@@ -402,8 +672,10 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
     for (int rt = 0; rt < 4; rt++)
 #pragma unroll
         for (int ct = 0; ct < 2; ct++) acc[rt][ct] = (v16i){0};
-    // two k-steps of prefetch: HBM latency spans about two k-steps of MFMA work
-    v4i pa[2], pa1[2], yb[2][2], y1[2][2], y2[2][2];
+    // Mc (L2-resident) is loaded one step ahead; Y two steps ahead into three
+    // register sets that rotate by unrolling the loop by three, so that no
+    // copy waits on a load just issued.
+    v4i pa[2], y0[2][2], y1[2][2], y2[2][2];
     uint64_t gfix[2] = {0, 0};
     int gcnt[2] = {0, 0};
     // A: 128 rows x 64 bytes of Mc, two 16-byte chunks per thread
@@ -412,73 +684,62 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
         const int ch = tid + 256 * u;                                                                  \
         dst[u] = *(const v4i *)(Mc + (int64_t)(rb * RECON_ROWS + (ch >> 2)) * Kr + (k0_) + 16 * (ch & 3)); \
     }
-    // B fragments straight from Y: K index 4q+s of an entry is byte s of Y[q][e].
-    // Columns 4q+s of Mc are zero for q >= P, so those planes are read
-    // clamped to the last prime (in bounds, multiplied by zero).
+    // B fragments straight from Y: K index 4w+b of an entry is byte b of word
+    // plane w.  Columns k >= S*P of Mc are zero, so planes past the last are
+    // read clamped to it (in bounds, multiplied by zero).
+    const int nyp = (S * P + 3) / 4;
 #define YLOAD(dst, k0_)                                                                  \
     _Pragma("unroll") for (int ct = 0; ct < 2; ct++)                                     \
     _Pragma("unroll") for (int kk = 0; kk < 2; kk++)                                     \
     _Pragma("unroll") for (int i = 0; i < 4; i++) {                                      \
-        const int q = min((k0_) / 4 + 8 * kk + 4 * h + i, P - 1);                        \
-        dst[ct][kk][i] = (int)yp[(int64_t)q * ystride + 32 * ct];                        \
+        const int w = min((k0_) / 4 + 8 * kk + 4 * h + i, nyp - 1);                      \
+        dst[ct][kk][i] = (int)yp[(int64_t)w * ystride + 32 * ct];                        \
+    }
+#define RSTEP(ycur, ynext2)                                                                          \
+    {                                                                                                \
+        __syncthreads();                                                                             \
+        _Pragma("unroll") for (int u = 0; u < 2; u++) {                                              \
+            const int ch = tid + 256 * u;                                                            \
+            *(v4i *)(LA + (ch >> 2) * LDS_PITCH + 16 * (ch & 3)) = pa[u];                            \
+        }                                                                                            \
+        __syncthreads();                                                                             \
+        if (k0 + 64 < Kr) { ALOAD(pa, k0 + 64); }                                                    \
+        if (k0 + 128 < Kr) { YLOAD(ynext2, k0 + 128); }                                              \
+        if (first) {                                                                                 \
+            _Pragma("unroll") for (int kk = 0; kk < 2; kk++)                                         \
+            _Pragma("unroll") for (int i = 0; i < 4; i++)                                            \
+            _Pragma("unroll") for (int sub = 0; sub < 4 / S; sub++) {                                \
+                const int q = (4 / S) * (k0 / 4 + 8 * kk + 4 * h + i) + sub;                         \
+                if (q < P) {                                                                         \
+                    const uint32_t p = pc[q].p;                                                      \
+                    const uint64_t fx = pc[q].fx;                                                    \
+                    _Pragma("unroll") for (int ct = 0; ct < 2; ct++) {                               \
+                        const uint32_t wd = (uint32_t)ycur[ct][kk][i];                               \
+                        const int32_t yc = S == 4 ? unslice4(wd) : unslice2(wd >> (16 * sub));       \
+                        const uint32_t y = yc < 0 ? (uint32_t)(yc + (int32_t)p) : (uint32_t)yc;      \
+                        gfix[ct] += ((uint64_t)y * fx) >> 31;                                        \
+                        gcnt[ct] += yc < 0;                                                          \
+                    }                                                                                \
+                }                                                                                    \
+            }                                                                                        \
+        }                                                                                            \
+        _Pragma("unroll") for (int kk = 0; kk < 2; kk++)                                             \
+        _Pragma("unroll") for (int rt = 0; rt < 4; rt++) {                                           \
+            const v4i af = *(const v4i *)(LA + (rt * 32 + c32) * LDS_PITCH + kk * 32 + 16 * h);      \
+            _Pragma("unroll") for (int ct = 0; ct < 2; ct++) acc[rt][ct] = mfma32(af, ycur[ct][kk], acc[rt][ct]); \
+        }                                                                                            \
+        k0 += 64;                                                                                    \
+        if (k0 >= Kr) break;                                                                         \
     }
     ALOAD(pa, 0);
-    YLOAD(yb, 0);
-    if (64 < Kr) {
-        ALOAD(pa1, 64);
-        YLOAD(y1, 64);
+    YLOAD(y0, 0);
+    if (64 < Kr) { YLOAD(y1, 64); }
+    for (int k0 = 0;;) {
+        RSTEP(y0, y2);
+        RSTEP(y1, y0);
+        RSTEP(y2, y1);
     }
-    for (int k0 = 0; k0 < Kr; k0 += 64) {
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < 2; u++) {
-            const int ch = tid + 256 * u;
-            *(v4i *)(LA + (ch >> 2) * LDS_PITCH + 16 * (ch & 3)) = pa[u];
-        }
-        __syncthreads();
-        if (k0 + 64 < Kr) {
-#pragma unroll
-            for (int u = 0; u < 2; u++) pa[u] = pa1[u];
-        }
-        if (k0 + 128 < Kr) {
-            ALOAD(pa1, k0 + 128);
-            YLOAD(y2, k0 + 128);
-        }
-        if (first) {
-#pragma unroll
-            for (int kk = 0; kk < 2; kk++)
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int q = k0 / 4 + 8 * kk + 4 * h + i;
-                    if (q < P) {
-                        const uint32_t p = pc[q].p;
-                        const uint64_t fx = pc[q].fx;
-#pragma unroll
-                        for (int ct = 0; ct < 2; ct++) {
-                            const int32_t yc = unslice4((uint32_t)yb[ct][kk][i]);
-                            const uint32_t y = yc < 0 ? (uint32_t)(yc + (int32_t)p) : (uint32_t)yc;
-                            gfix[ct] += ((uint64_t)y * fx) >> 31;
-                            gcnt[ct] += yc < 0;
-                        }
-                    }
-                }
-        }
-#pragma unroll
-        for (int kk = 0; kk < 2; kk++)
-#pragma unroll
-            for (int rt = 0; rt < 4; rt++) {
-                const v4i af = *(const v4i *)(LA + (rt * 32 + c32) * LDS_PITCH + kk * 32 + 16 * h);
-#pragma unroll
-                for (int ct = 0; ct < 2; ct++) acc[rt][ct] = mfma32(af, yb[ct][kk], acc[rt][ct]);
-            }
-#pragma unroll
-        for (int ct = 0; ct < 2; ct++)
-#pragma unroll
-            for (int kk = 0; kk < 2; kk++) {
-                yb[ct][kk] = y1[ct][kk];
-                y1[ct][kk] = y2[ct][kk];
-            }
-    }
+#undef RSTEP
 #undef ALOAD
 #undef YLOAD
     // row 8g + 4h + u of row tile rt is byte u of word rb*32 + rt*8 + 2g + h
@@ -592,7 +853,7 @@ struct PlanView {
     const PrimeConst *pc;
     const int8_t *cst, *mc;
     const uint32_t *m32;
-    int64_t P, kb, Kr, rows4pad, Lbp, out_limbs, nw;
+    int64_t S, P, kb, Kr, rows4pad, Lbp, out_limbs, nw;
 };
 int plan_view(const mgx_crt_plan *pl, PlanView *v);
 int cached_plan(int64_t K, int64_t bitsA, int64_t bitsB, const mgx_crt_plan **out);

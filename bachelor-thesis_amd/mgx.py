@@ -102,6 +102,8 @@ def _declare(L):
     L.mgx_crt_plan_primes.restype = c_i64
     L.mgx_crt_y_stride.argtypes = [c_i64, c_i64]
     L.mgx_crt_y_stride.restype = c_i64
+    L.mgx_crt_plan_slices.argtypes = [c_vp]
+    L.mgx_crt_plan_slices.restype = c_i64
 
 
 def check(status):

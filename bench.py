@@ -78,11 +78,12 @@ def main():
     # ---- stage kernels timed with HIP events on the stream they run on
     plan = mgx.crt_plan(n, bits, bits)
     P = L.mgx_crt_plan_primes(plan)
+    S = L.mgx_crt_plan_slices(plan)
     rp, kp = mgx.c_i64(), mgx.c_i64()
     mgx.call("mgx_crt_residue_dims", plan, n, n, ctypes.byref(rp), ctypes.byref(kp))
     rpad, kpad = rp.value, kp.value
-    RA = mgx.DeviceBuffer(4 * P * rpad * kpad)
-    RB = mgx.DeviceBuffer(4 * P * rpad * kpad)
+    RA = mgx.DeviceBuffer(S * P * rpad * kpad)
+    RB = mgx.DeviceBuffer(S * P * rpad * kpad)
     ystride = L.mgx_crt_y_stride(n, n)
     Y = mgx.DeviceBuffer(P * ystride * 4)
     ev = [mgx.c_vp() for _ in range(5)]
@@ -108,7 +109,7 @@ def main():
             mgx.call("mgx_event_elapsed", ctypes.byref(ms), ev[k].value, ev[k + 1].value)
             stage_ms[k] += ms.value / reps
     log("stages ms: reduceA %.3f reduceB %.3f gemm %.3f reconstruct %.3f (P=%d)" % tuple(stage_ms + [P]))
-    gemm_ops = 2.0 * n ** 3 * 16 * P
+    gemm_ops = 2.0 * n ** 3 * S * S * P
     gemm_tops = gemm_ops / (stage_ms[2] * 1e-3) / 1e12
     C = dC.to_array((n, n, W))
 
@@ -185,18 +186,18 @@ def main():
         "vs_baseline": None,
         "dtype": "int8 MFMA / int64",
         "data": "synthetic: seeded xoshiro256** entries, uniform %d-bit magnitudes, random signs" % bits,
-        "config": {"workload": "C=A*B, n=%d square, signed %d-bit entries, CRT-MFMA route (P=%d primes < 2^31, "
-                               "4 int8 slices)" % (n, bits, P),
-                   "n": n, "bits": bits, "primes": P, "cap_limbs": cap},
+        "config": {"workload": "C=A*B, n=%d square, signed %d-bit entries, CRT-MFMA route (P=%d primes < %s, "
+                               "%d int8 slices)" % (n, bits, P, "65280" if S == 2 else "2^31", S),
+                   "n": n, "bits": bits, "primes": P, "slices": S, "cap_limbs": cap},
         "entry_products_per_s": n ** 3 / crt_s,
         "plain_route_s": plain_s,
         "speedup_vs_plain": (plain_s / crt_s) if plain_s else None,
         "bit_exact": {"crt_vs_plain_all_entries": bit_exact_routes, "crt_vs_oracle_sample": sample_ok},
         "stage_ms": {"reduce_A": stage_ms[0], "reduce_B": stage_ms[1], "residue_gemm": stage_ms[2],
                      "reconstruct": stage_ms[3]},
-        "roofline": {"bound": "mfma", "kernel": "crt_gemm_kernel", "achieved": gemm_tops, "peak": INT8_PEAK_TOPS,
+        "roofline": {"bound": "mfma", "kernel": "crt_gemm2_kernel" if S == 2 else "crt_gemm_kernel", "achieved": gemm_tops, "peak": INT8_PEAK_TOPS,
                      "unit": "TFLOP/s", "frac": gemm_tops / INT8_PEAK_TOPS, "traffic": None,
-                     "note": "int8 ops 2*n^3*16*P per launch / event-timed launch duration"},
+                     "note": "int8 ops 2*n^3*S^2*P per launch / event-timed launch duration"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

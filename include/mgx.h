@@ -175,15 +175,18 @@ int mgx_matmul_host(int method, int64_t K, int64_t M, int64_t N, mgx_limb *C, co
 /* ---------------- multi-modular CRT route on INT8 MFMA (new; no reference counterpart) ---------------- */
 typedef struct mgx_crt_plan mgx_crt_plan;
 /* Plan for products with inner dimension K and |A| < 2^bitsA, |B| < 2^bitsB:
- * the smallest prefix of the primes below 2^31 (descending) whose product
- * exceeds 2^(bitsA + bitsB + ceil(log2 K) + 2), with all per-prime constants
- * resident on the device.  Synchronous. */
+ * the smallest prefix of the primes below 65280 (S = 2 digits per residue;
+ * the default while the primes above 16384 suffice) or below 2^31 (S = 4),
+ * descending, whose product exceeds 2^(bitsA + bitsB + ceil(log2 K) + 2),
+ * with all per-prime constants resident on the device (MGX_PARAM_CRT_SLICES
+ * forces S).  Synchronous. */
 int mgx_crt_plan_create(mgx_crt_plan **plan, int64_t K, int64_t bitsA, int64_t bitsB);
 /* Largest bit length of |x| over a rows x cols matrix (synchronizes). */
 int mgx_crt_scan_bits(int64_t rows, int64_t cols, const mgx_limb *X, int64_t Xrs, int64_t cap, int64_t *bits,
                       void *stream);
 int mgx_crt_plan_destroy(mgx_crt_plan *plan);
 int64_t mgx_crt_plan_primes(const mgx_crt_plan *plan);
+int64_t mgx_crt_plan_slices(const mgx_crt_plan *plan);   /* S: 2 (primes < 65280) or 4 (primes < 2^31) */
 int mgx_crt_plan_prime_list(const mgx_crt_plan *plan, uint32_t *out);
 /* C = A*B through the plan (plan NULL: scan the inputs' bit lengths on the
  * device, which synchronizes the stream once, and plan internally). */
@@ -191,12 +194,14 @@ int mgx_crt_mfma_matmul(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t 
                         const mgx_limb *A, int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap,
                         void *stream);
 /* Stage kernels, exposed for testing and profiling.  Residues are planar
- * int8 slices R[q][s][row][k] with row pitch kpad and rows padded to rpad
- * (mgx_crt_residue_dims).  Y is [q][ystride] uint32 with ystride =
- * mgx_crt_y_stride(K, N) (K*N rounded up to 256): entry i*N+j of plane q holds
- * y_q = (C_ij mod p_q) (M/p_q)^-1 mod p_q, centred and written as four
- * balanced int8 digits (byte s = digit s).  mgx_crt_reconstruct takes its own
- * stream-ordered scratch (about 8 * ystride bytes per 32-bit word of M). */
+ * int8 slices R[q][s][row][k] (s < S) with row pitch kpad and rows padded to
+ * rpad (mgx_crt_residue_dims).  Y holds y_q = (C_ij mod p_q) (M/p_q)^-1 mod
+ * p_q, centred and written as S balanced int8 digits, at entry i*N+j of
+ * uint32 word planes of ystride = mgx_crt_y_stride(K, N) words (K*N rounded
+ * up to 256): plane q for S = 4 (byte s = digit s); for S = 2 plane q/2,
+ * low half for even q, high half for odd q (ceil(P/2) planes).
+ * mgx_crt_reconstruct takes its own stream-ordered scratch (about
+ * 8 * ystride bytes per 32-bit word of M). */
 int mgx_crt_residue_dims(const mgx_crt_plan *plan, int64_t rows, int64_t K, int64_t *rpad, int64_t *kpad);
 int mgx_crt_reduce(const mgx_crt_plan *plan, int transpose, int64_t rows, int64_t K, const mgx_limb *X,
                    int64_t Xrs, int64_t cap, int64_t nbytes, int8_t *R, void *stream);
@@ -226,7 +231,9 @@ int mgx_crt_merge_list(mgx_limb *x, const mgx_limb *split, const mgx_limb *ms, c
 typedef enum {
     MGX_PARAM_LEAF = 0,          /* recursion leaf size (reference BASE_SIZE = 256) */
     MGX_PARAM_TOOM22 = 1,        /* Toom-22 block cutoff in 64-limb blocks (0 = never) */
-    MGX_PARAM_WINOGRAD_DEPTH = 2 /* max recursion levels for the plain route (-1 = to the leaf) */
+    MGX_PARAM_WINOGRAD_DEPTH = 2, /* max recursion levels for the plain route (-1 = to the leaf) */
+    MGX_PARAM_CRT_SLICES = 3      /* CRT digits per residue for new plans: 0 = 2 when primes below 65280
+                                     suffice, else 4 (default); 2 or 4 to force */
 } mgx_param;
 int mgx_set_param(int param, int64_t value);
 int64_t mgx_get_param(int param);

@@ -46,10 +46,11 @@ int or_is_prime_u32(uint64_t n)
     return 1;
 }
 
-/* The count largest primes below 2^31, descending. */
-int or_crt_primes(int64_t count, uint32_t *out)
+/* The count largest primes <= limit, descending (crt.hip primes_for_bits:
+ * limit 2^31 - 1 for four digits per residue, 65279 for two). */
+int or_crt_primes_below(uint64_t limit, int64_t count, uint32_t *out)
 {
-    uint64_t c = (1ull << 31) - 1;
+    uint64_t c = limit;
     for (int64_t i = 0; i < count; c--) {
         if (c < 3) return 1;
         if (or_is_prime_u32(c)) out[i++] = (uint32_t)c;
@@ -57,15 +58,18 @@ int or_crt_primes(int64_t count, uint32_t *out)
     return 0;
 }
 
-/* Smallest P with prod(first P primes) > 2^bits_needed, by exact big-integer
- * comparison (a limb vector product). */
-int64_t or_crt_prime_count(int64_t bits_needed)
+int or_crt_primes(int64_t count, uint32_t *out) { return or_crt_primes_below((1ull << 31) - 1, count, out); }
+
+/* Smallest P with prod(first P primes <= limit) > 2^bits_needed, by exact
+ * big-integer comparison (a limb vector product); -1 if the primes above 2
+ * run out first. */
+int64_t or_crt_prime_count_below(uint64_t limit, int64_t bits_needed)
 {
     int64_t cap = bits_needed / 64 + 4;
     olimb *m = (olimb *)calloc((size_t)cap + 64, sizeof(olimb));
     int64_t n = 1; m[0] = 1;
     int64_t P = 0;
-    uint64_t c = (1ull << 31) - 1;
+    uint64_t c = limit;
     for (;;) {
         /* m > 2^bits_needed <=> bit length of m > bits_needed, or == and m is not a power of two */
         int64_t top = n - 1;
@@ -78,7 +82,8 @@ int64_t or_crt_prime_count(int64_t bits_needed)
             if (m[top] & (m[top] - 1)) pow2 = 0;
             if (!pow2) break;
         }
-        while (!or_is_prime_u32(c)) c--;
+        while (c >= 3 && !or_is_prime_u32(c)) c--;
+        if (c < 3) { P = -1; break; }
         m[n] = or_mpn_mul_1(m, m, n, c);
         n++;
         while (n > 1 && m[n - 1] == 0) n--;
@@ -88,6 +93,8 @@ int64_t or_crt_prime_count(int64_t bits_needed)
     free(m);
     return P;
 }
+
+int64_t or_crt_prime_count(int64_t bits_needed) { return or_crt_prime_count_below((1ull << 31) - 1, bits_needed); }
 
 /* Signed entry modulo p, in [0, p): Horner over the limbs, high to low. */
 uint32_t or_mod_entry(const olimb *e, int64_t cap, uint32_t p)

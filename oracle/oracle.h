@@ -81,6 +81,8 @@ int or_mpz_inp_str(olimb *e, int64_t cap, const char *str);
 
 /* ---- multi-modular route (mathematics; no reference counterpart) ---- */
 int or_crt_primes(int64_t count, uint32_t *out);             /* largest primes below 2^31, descending */
+int or_crt_primes_below(uint64_t limit, int64_t count, uint32_t *out);  /* largest primes <= limit */
+int64_t or_crt_prime_count_below(uint64_t limit, int64_t bits_needed);   /* -1: not enough primes */
 int64_t or_crt_prime_count(int64_t bits_needed);              /* primes needed to exceed 2^bits_needed */
 uint32_t or_mod_entry(const olimb *e, int64_t cap, uint32_t p); /* signed entry mod p, in [0,p) */
 void or_balanced_slices(int32_t r, int8_t d[4]);              /* r = sum d_i 2^(8i), |r| < 2^30 */

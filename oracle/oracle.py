@@ -41,6 +41,8 @@ def lib():
             "or_mpz_inp_str": ([c_vp, c_i64, ctypes.c_char_p], ctypes.c_int),
             "or_crt_primes": ([c_i64, c_vp], ctypes.c_int),
             "or_crt_prime_count": ([c_i64], c_i64),
+            "or_crt_primes_below": ([ctypes.c_uint64, c_i64, c_vp], ctypes.c_int),
+            "or_crt_prime_count_below": ([ctypes.c_uint64, c_i64], c_i64),
             "or_mod_entry": ([c_vp, c_i64, ctypes.c_uint32], ctypes.c_uint32),
             "or_balanced_slices": ([ctypes.c_int32, c_vp], None),
             "or_crt_reconstruct": ([c_vp, c_i64, c_vp, c_vp, c_i64], ctypes.c_int),
@@ -99,9 +101,9 @@ def mat_aors(A, B, cap, sub=False):
     return C
 
 
-def primes(count):
+def primes(count, limit=(1 << 31) - 1):
     out = np.zeros(count, dtype=np.uint32)
-    assert lib().or_crt_primes(count, _p(out)) == 0
+    assert lib().or_crt_primes_below(limit, count, _p(out)) == 0
     return [int(x) for x in out]
 
 

@@ -91,7 +91,20 @@ def crt_vectors():
             v = (v - t) // 256
         assert v == 0 and sum(t * 256 ** i for i, t in enumerate(d)) == r
         digs.append({"r": r, "d": d})
-    return {"primes": ps, "prime_counts": counts, "residues": residues, "slices": digs}
+    # two-digit residues: the largest primes below 65280 and their prefix counts
+    ps16, p = [], 65280
+    while len(ps16) < 700:
+        p = sympy.prevprime(p)
+        ps16.append(p)
+    counts16 = {}
+    for bits in [1, 15, 100, 1000, 2048 * 2 + 10 + 2, 4096 * 2 + 8 + 2]:
+        m, n = 1, 0
+        while m <= (1 << bits):
+            m *= ps16[n]
+            n += 1
+        counts16[str(bits)] = n
+    return {"primes": ps, "prime_counts": counts, "residues": residues, "slices": digs,
+            "primes16": ps16, "prime_counts16": counts16}
 
 
 def text_case():

@@ -20,6 +20,14 @@ def run_crt(mgx, A, B, cap, plan=None):
     return dC.to_array((K, N, cap + 1))
 
 
+@pytest.fixture(params=[2, 4], ids=["S2", "S4"])
+def slices(request, gpu):
+    """Digits per residue for the plans a test makes: 2 (primes below 65280) or 4 (below 2^31)."""
+    gpu.call("mgx_set_param", 3, request.param)
+    yield request.param
+    gpu.call("mgx_set_param", 3, 0)
+
+
 def product_plan_M(primes):
     M = 1
     for p in primes:
@@ -28,7 +36,7 @@ def product_plan_M(primes):
 
 
 @pytest.mark.parametrize("transpose", [0, 1])
-def test_reduce_kernel(gpu, transpose):
+def test_reduce_kernel(gpu, slices, transpose):
     mgx = gpu
     rows, K, bits, cap = 70, 45, 700, 14
     vals = rand_ints(rows, K, bits, 3 + transpose)
@@ -37,58 +45,70 @@ def test_reduce_kernel(gpu, transpose):
     plan = mgx.crt_plan(K, bits, bits)
     try:
         ps = mgx.crt_plan_primes(plan)
+        S = mgx.lib().mgx_crt_plan_slices(plan)
+        assert S == slices
+        limit = 65279 if S == 2 else (1 << 31) - 1
+        assert ps == oracle.primes(len(ps), limit=limit)
+        assert len(ps) == oracle.lib().or_crt_prime_count_below(limit, 2 * bits + 6 + 2)
         rp, kp = mgx.c_i64(), mgx.c_i64()
         mgx.call("mgx_crt_residue_dims", plan, rows, K, mgx.ctypes.byref(rp), mgx.ctypes.byref(kp))
         rpad, kpad = rp.value, kp.value
         dX = mgx.DeviceBuffer.from_array(X)
-        dR = mgx.DeviceBuffer(4 * len(ps) * rpad * kpad)
+        dR = mgx.DeviceBuffer(S * len(ps) * rpad * kpad)
         xrs = X.shape[1]
         mgx.call("mgx_crt_reduce", plan, transpose, rows, K, dX.ptr, xrs, cap, 0, dR.ptr, None)
         mgx.call("mgx_check_error", None)
-        R = dR.to_array((len(ps), 4, rpad, kpad), dtype=np.int8)
+        R = dR.to_array((len(ps), S, rpad, kpad), dtype=np.int8)
         for qi in [0, 1, len(ps) // 2, len(ps) - 1]:
             p = ps[qi]
             for r in range(rows):
                 for k in range(K):
                     res = vals[r][k] % p
                     cen = res - p if res > p // 2 else res
-                    assert [int(R[qi, s, r, k]) for s in range(4)] == oracle.slices(cen), (qi, r, k)
+                    d = oracle.slices(cen)   # four digits; the top S..3 vanish when |cen| < 2^15
+                    assert d[S:] == [0] * (4 - S)
+                    assert [int(R[qi, s, r, k]) for s in range(S)] == d[:S], (qi, r, k)
             assert not R[qi, :, rows:, :].any() and not R[qi, :, :, K:].any()   # zero padding
     finally:
         mgx.call("mgx_crt_plan_destroy", plan)
 
 
-def test_residue_gemm_kernel(gpu):
+def test_residue_gemm_kernel(gpu, slices):
     mgx = gpu
     K, M, N, bits, cap = 37, 50, 29, 260, 10
     A, B = rand_ints(K, M, bits, 21), rand_ints(M, N, bits, 22)
     plan = mgx.crt_plan(M, bits, bits)
     try:
         ps = mgx.crt_plan_primes(plan)
+        S = mgx.lib().mgx_crt_plan_slices(plan)
         Mprod = product_plan_M(ps)
         rp, kp, rpb = mgx.c_i64(), mgx.c_i64(), mgx.c_i64()
         mgx.call("mgx_crt_residue_dims", plan, K, M, mgx.ctypes.byref(rp), mgx.ctypes.byref(kp))
         mgx.call("mgx_crt_residue_dims", plan, N, M, mgx.ctypes.byref(rpb), None)
         a, b = to_entries(A, cap), to_entries(B, cap)
         dA, dB = mgx.DeviceBuffer.from_array(a), mgx.DeviceBuffer.from_array(b)
-        dRA = mgx.DeviceBuffer(4 * len(ps) * rp.value * kp.value)
-        dRB = mgx.DeviceBuffer(4 * len(ps) * rpb.value * kp.value)
+        dRA = mgx.DeviceBuffer(S * len(ps) * rp.value * kp.value)
+        dRB = mgx.DeviceBuffer(S * len(ps) * rpb.value * kp.value)
         ystride = mgx.lib().mgx_crt_y_stride(K, N)
         assert ystride == (K * N + 255) // 256 * 256
-        dY = mgx.DeviceBuffer(len(ps) * ystride * 4)
+        nplanes = (S * len(ps) + 3) // 4
+        dY = mgx.DeviceBuffer(nplanes * ystride * 4)
         mgx.call("mgx_crt_reduce", plan, 0, K, M, dA.ptr, M, cap, 0, dRA.ptr, None)
         mgx.call("mgx_crt_reduce", plan, 1, N, M, dB.ptr, N, cap, 0, dRB.ptr, None)
         mgx.call("mgx_crt_residue_gemm", plan, K, M, N, dRA.ptr, dRB.ptr, dY.ptr, None)
         mgx.call("mgx_check_error", None)
-        Y = dY.to_array((len(ps), ystride), dtype=np.uint32)
+        Y = dY.to_array((nplanes, ystride), dtype=np.uint32)
         C = matmul(A, B)
         for qi in [0, 3, len(ps) - 1]:
             p = ps[qi]
             inv = pow((Mprod // p) % p, p - 2, p)
             for i in range(K):
                 for j in range(N):
-                    w = int(Y[qi, i * N + j])
-                    d = [((w >> (8 * s)) & 255) - (256 if (w >> (8 * s)) & 128 else 0) for s in range(4)]
+                    if S == 4:
+                        w = int(Y[qi, i * N + j])
+                    else:   # plane q/2, low half for even q, high half for odd q
+                        w = (int(Y[qi // 2, i * N + j]) >> (16 * (qi % 2))) & 0xffff
+                    d = [((w >> (8 * s)) & 255) - (256 if (w >> (8 * s)) & 128 else 0) for s in range(S)]
                     y = sum(t * 256 ** s for s, t in enumerate(d)) % p
                     assert y == (C[i][j] % p) * inv % p, (qi, i, j)
     finally:
@@ -100,7 +120,7 @@ SHAPES = [(1, 1, 1), (1, 5, 1), (3, 4, 5), (7, 1, 9), (33, 17, 65), (64, 64, 64)
 
 @pytest.mark.parametrize("K,M,N", SHAPES)
 @pytest.mark.parametrize("bits", [1, 64, 200, 1024])
-def test_crt_matmul_small(gpu, K, M, N, bits):
+def test_crt_matmul_small(gpu, slices, K, M, N, bits):
     if bits == 1024 and K * M * N > 64 * 64 * 64:
         pytest.skip("oracle time")
     A, B = rand_ints(K, M, bits, K * 7 + bits), rand_ints(M, N, bits, N * 11 + bits)
@@ -111,7 +131,7 @@ def test_crt_matmul_small(gpu, K, M, N, bits):
     assert is_normalized(C)
 
 
-def test_crt_edge_values(gpu):
+def test_crt_edge_values(gpu, slices):
     # zeros, +-1, all-ones magnitudes, exact cancellation to zero, one-sided signs
     bits, cap = 2048, 66
     top = (1 << bits) - 1
@@ -122,15 +142,16 @@ def test_crt_edge_values(gpu):
     assert is_normalized(C)
 
 
-def test_crt_long_inner_dimension(gpu):
-    # K beyond one int32 accumulation chunk (8192) takes the folding variant
-    K, M, N, bits, cap = 3, 8300, 2, 100, 6
+@pytest.mark.parametrize("M", [8300, 33000])
+def test_crt_long_inner_dimension(gpu, slices, M):
+    # K beyond one int32 accumulation chunk (8192 for S = 4, 32768 for S = 2) takes the folding variant
+    K, N, bits, cap = 3, 2, 100, 6
     A, B = rand_ints(K, M, bits, 1), rand_ints(M, N, bits, 2)
     C = run_crt(gpu, to_entries(A, cap), to_entries(B, cap), cap)
     assert to_ints(C) == matmul(A, B)
 
 
-def test_crt_capacity_error(gpu):
+def test_crt_capacity_error(gpu, slices):
     mgx = gpu
     A = [[(1 << 300) - 1]]
     cap = 6   # the product needs 10 limbs
@@ -139,7 +160,7 @@ def test_crt_capacity_error(gpu):
     assert ei.value.status == 1
 
 
-def test_crt_explicit_plan_reused(gpu):
+def test_crt_explicit_plan_reused(gpu, slices):
     mgx = gpu
     plan = mgx.crt_plan(40, 500, 300)
     try:
@@ -151,14 +172,14 @@ def test_crt_explicit_plan_reused(gpu):
         mgx.call("mgx_crt_plan_destroy", plan)
 
 
-def test_crt_full_size_properties(gpu):
+def test_crt_full_size_properties(gpu, slices):
     """n=1024, 2048-bit entries: an independent modulus, sampled exact entries."""
     mgx = gpu
     n, bits, cap = 1024, 2048, 66
     A, B = mgx.random_matrix(n, n, cap, bits, 1), mgx.random_matrix(n, n, cap, bits, 2)
     C = run_crt(mgx, A, B, cap)
     assert is_normalized(C[:8])
-    q = 67108859   # largest prime below 2^26, not a CRT prime (those are above 2^30)
+    q = 67108859   # largest prime below 2^26, not a CRT prime (those are below 2^16 or above 2^30)
 
     def mod_q(X):
         r = np.zeros(X.shape[:2], dtype=np.uint64)

@@ -168,6 +168,12 @@ def test_crt_primes_and_counts(orc):
     assert oracle.primes(len(ps)) == ps
     for bits, n in kat["prime_counts"].items():
         assert orc.or_crt_prime_count(int(bits)) == n
+    # two-digit residues: primes below 65280
+    ps16 = kat["primes16"]
+    assert oracle.primes(len(ps16), limit=65279) == ps16
+    for bits, n in kat["prime_counts16"].items():
+        assert orc.or_crt_prime_count_below(65279, int(bits)) == n
+    assert orc.or_crt_prime_count_below(65279, 200000) == -1   # not enough 16-bit primes
     # Miller-Rabin against trial division on small odd numbers
     for n in range(3, 4000, 2):
         assert bool(orc.or_is_prime_u32(n)) == all(n % d for d in range(2, int(n ** 0.5) + 1))
