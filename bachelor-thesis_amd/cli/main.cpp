@@ -16,20 +16,23 @@
 #include <string>
 #include <vector>
 #include "mgx.h"
+#include "mgx_sched.h"
 
 static int usage(const char *argv0)
 {
     fprintf(stderr,
-            "Usage: %s [-hip] [-n size] [-b nbits] [-m method] [-s seed] [-j]\n"
+            "Usage: %s [-hip] [-n size] [-b nbits] [-m method] [-s seed] [-j] [-M model]\n"
             "  -h         Show help\n"
             "  -i         Read matrices from input instead of generating randomly\n"
             "  -n K,M,N   Set the size for random matrix generation (KxM * MxN = KxN)\n"
             "  -b nbits   Number of bits for generated numbers\n"
             "  -m method  1/2 naive, 3 Strassen, 35 Winograd, 4 naive recursive,\n"
-            "             37/357/47 interleaved Strassen/Winograd/naive, 8 CRT-MFMA\n"
+            "             37/357/47 interleaved Strassen/Winograd/naive, 8 CRT-MFMA,\n"
+            "             0 the route the cost model picks\n"
             "  -p         Print result after multiplication\n"
             "  -s seed    Seed for generated numbers (default: the time)\n"
-            "  -j         Print a JSON line with the timing\n",
+            "  -j         Print a JSON line with the timing\n"
+            "  -M model   Cost model for -m 0 (default: costmodel_params.txt beside bin/)\n",
             argv0);
     return 0;
 }
@@ -51,6 +54,7 @@ int main(int argc, char **argv)
     long long K = -1, M = -1, N = -1, nbits = -1;
     int method = 1;
     unsigned long long seed = (unsigned long long)time(nullptr);
+    std::string model_path;
     for (int i = 1; i < argc; i++) {
         const char *a = argv[i];
         if (a[0] != '-' || !a[1]) { fprintf(stderr, "Unknown flag, use -h to get info\n"); return 1; }
@@ -75,6 +79,7 @@ int main(int argc, char **argv)
             } else if (f == 'b') { const char *v = value(); if (!v) return 1; nbits = atoll(v); break; }
             else if (f == 'm') { const char *v = value(); if (!v) return 1; method = atoi(v); break; }
             else if (f == 's') { const char *v = value(); if (!v) return 1; seed = strtoull(v, nullptr, 10); break; }
+            else if (f == 'M') { const char *v = value(); if (!v) return 1; model_path = v; break; }
             else { fprintf(stderr, "Unknown flag, use -h to get info\n"); return 1; }
         }
     }
@@ -128,9 +133,24 @@ int main(int argc, char **argv)
             return 1;
         }
     }
+    mgx_costmodel *model = nullptr;
+    if (method == 0) {   // learned dispatch: install the model, else the autotuner times the routes
+        if (model_path.empty()) {
+            std::string exe = argv[0];
+            size_t slash = exe.rfind('/');
+            model_path = (slash == std::string::npos ? std::string(".") : exe.substr(0, slash)) +
+                         "/../costmodel_params.txt";
+        }
+        if (mgx_costmodel_load(&model, model_path.c_str()) == MGX_OK) mgx_dispatch_set_model(model);
+        else fprintf(stderr, "No cost model (%s); timing the routes instead\n", mgx_last_error());
+    }
     fprintf(stderr, "Using method %d\n", method);
     double secs = 0;
     int st = mgx_matmul_host(method, K, M, N, C.data(), A.data(), B.data(), cap, &secs);
+    if (model) {
+        mgx_dispatch_set_model(nullptr);
+        mgx_costmodel_destroy(model);
+    }
     if (st != MGX_OK) { fprintf(stderr, "Multiplication failed: %s\n", mgx_last_error()); return 1; }
     fprintf(stderr, "Time taken: %g seconds\n", secs);
     if (print_result) {

@@ -1,6 +1,7 @@
 """Train and evaluate the learned route dispatcher.
 
-    python bachelor-thesis_amd/costmodel.py train  [--configs 5000] [--out costmodel_params.json]
+    python bachelor-thesis_amd/costmodel.py gen    --data timings.npz [--configs 5000]
+    python bachelor-thesis_amd/costmodel.py train  [--data timings.npz] [--configs 5000] [--out costmodel_params.json]
     python bachelor-thesis_amd/costmodel.py eval   [--params costmodel_params.json] [--json out.json]
 
 `train` times every route (mgx_tune_time_routes: the exhaustive autotuner) on
@@ -110,12 +111,20 @@ def sample_config(rng):
     return K, M, N, blo, bhi, frac
 
 
-def generate(nconfigs, seed, log_every=100):
+def generate(nconfigs, seed, log_every=100, save=None, start=None):
+    """Time every route on nconfigs seeded problems.  With `save`, the arrays
+    gathered so far are written there every log_every configs, and a run
+    resumes from such a file (`start` = its arrays)."""
     rng = np.random.default_rng(seed)
     X, T, flags, cfgs = [], [], [], []
+    if start is not None:
+        X, T, flags, cfgs = [list(a) for a in start]
     t0 = time.time()
     for i in range(nconfigs):
-        K, M, N, blo, bhi, frac = sample_config(rng)
+        cfg = sample_config(rng)   # drawn for every i, so a resumed run sees the same sequence
+        if i < len(X):
+            continue
+        K, M, N, blo, bhi, frac = cfg
         p = Problem(K, M, N, blo, bhi, frac, 1000 + i)
         X.append(p.features())
         t, est = p.times(reps=1)
@@ -125,6 +134,9 @@ def generate(nconfigs, seed, log_every=100):
         p.free()
         if (i + 1) % log_every == 0:
             print("timed %d/%d configs (%.0f s)" % (i + 1, nconfigs, time.time() - t0), file=sys.stderr, flush=True)
+            if save:
+                np.savez(save, X=np.array(X, dtype=np.float32), T=np.array(T), flags=np.array(flags),
+                         cfgs=np.array(cfgs, dtype=np.float64))
     return np.array(X, dtype=np.float32), np.array(T), np.array(flags), cfgs
 
 
@@ -156,6 +168,11 @@ def save_model(cm, mean, scale, path, meta):
     with open(path, "w") as f:
         json.dump({"D": D, "H": H, "O": O, "routes": ROUTES, "mean": mean.tolist(), "scale": scale.tolist(),
                    "params": p.tolist(), "meta": meta}, f)
+    # the same numbers as text for mgx_costmodel_load (C callers, mat-gpu -m 0)
+    with open(os.path.splitext(path)[0] + ".txt", "w") as f:
+        f.write("mgx-costmodel 1 %d %d %d\n" % (D, H, O))
+        for arr in (mean, scale, p):
+            f.write(" ".join("%.9g" % float(x) for x in arr) + "\n")
 
 
 def load_model(path=DEFAULT_PARAMS):
@@ -179,7 +196,8 @@ def predict(cm, X):
 
 
 HELD_OUT = [
-    # (name, K, M, N, bits_lo, bits_hi, frac_hi)
+    # (name, K, M, N, bits_lo, bits_hi, frac_hi); the last four are small
+    # products where the per-entry plain route beats the CRT's fixed costs
     ("n768_90pc256_10pc32768", 768, 768, 768, 256, 32768, 0.10),
     ("n1024_2048", 1024, 1024, 1024, 2048, 2048, 0.0),
     ("n64_64bit", 64, 64, 64, 64, 64, 0.0),
@@ -190,6 +208,10 @@ HELD_OUT = [
     ("n400_99pc64_1pc16384", 400, 400, 400, 64, 16384, 0.01),
     ("n16_16384", 16, 16, 16, 16384, 16384, 0.0),
     ("n300_512", 300, 300, 300, 512, 512, 0.0),
+    ("n19_1600", 19, 19, 19, 1600, 1600, 0.0),
+    ("n36_157_1pc5763", 36, 36, 36, 157, 5763, 0.01),
+    ("n20_391_10pc10466", 20, 20, 20, 391, 10466, 0.10),
+    ("n28_rect_187_5pc1899", 28, 56, 28, 187, 1899, 0.05),
 ]
 
 
@@ -225,7 +247,7 @@ def evaluate(cm, reps=3, check_exact=True):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("cmd", choices=["train", "eval"])
+    ap.add_argument("cmd", choices=["gen", "train", "eval"])
     ap.add_argument("--configs", type=int, default=5000)
     ap.add_argument("--epochs", type=int, default=3000)
     ap.add_argument("--seed", type=int, default=2024)
@@ -235,7 +257,16 @@ def main():
     ap.add_argument("--json", default=None)
     a = ap.parse_args()
     mgx.init(0)
-    if a.cmd == "train":
+    if a.cmd == "gen":
+        start = None
+        if a.data and os.path.exists(a.data):
+            z = np.load(a.data)
+            start = (z["X"], z["T"], z["flags"], z["cfgs"])
+            print("resuming after %d configs" % len(z["X"]), file=sys.stderr, flush=True)
+        X, T, flags, cfgs = generate(a.configs, a.seed, save=a.data, start=start)
+        np.savez(a.data, X=X, T=T, flags=flags, cfgs=np.array(cfgs, dtype=np.float64))
+        print(json.dumps({"configs": int(X.shape[0]), "data": a.data}))
+    elif a.cmd == "train":
         t0 = time.time()
         if a.data and os.path.exists(a.data):
             z = np.load(a.data)

@@ -117,21 +117,21 @@ template <int S>
 __global__ __launch_bounds__(256, 2) void crt_recon_gemm_kernel(
     const PrimeConst *__restrict__ pc, int P, const int8_t *__restrict__ Mc, int Kr, int nrb, int nw,
     const uint32_t *__restrict__ Y, int64_t ystride, int64_t nent, int64_t *__restrict__ Wv,
-    int32_t *__restrict__ gamma, int64_t total)
+    uint64_t *__restrict__ G, int64_t total)
 {
     // XCD-aware order: the row blocks of one entry block share an L2
     int64_t per = (total + 7) / 8;
     int64_t v = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
     if (v >= total) return;
-    recon_gemm_tile<S>(v, pc, P, Mc, Kr, nrb, nw, Y, ystride, nent, Wv, gamma);
+    recon_gemm_tile<S>(v, pc, P, Mc, Kr, nrb, nw, Y, ystride, nent, Wv, G);
 }
 
 __global__ __launch_bounds__(256) void crt_recon_carry_kernel(
-    const int64_t *__restrict__ Wv, int64_t ystride, int nw, const int32_t *__restrict__ gamma,
+    const int64_t *__restrict__ Wv, int64_t ystride, int nw, int nrb, const uint64_t *__restrict__ G,
     const uint32_t *__restrict__ M32, int64_t nent, int64_t colsC, uint64_t *__restrict__ C, int64_t Crs, int64_t W,
     int64_t cap, unsigned *__restrict__ err)
 {
-    recon_carry_block(blockIdx.x, Wv, ystride, nw, gamma, M32, nent, colsC, C, Crs, W, cap, err);
+    recon_carry_block(blockIdx.x, Wv, ystride, nw, nrb, G, M32, nent, colsC, C, Crs, W, cap, err);
 }
 
 }  // namespace mgx
@@ -462,21 +462,23 @@ int mgx_crt_residue_gemm(const mgx_crt_plan *pl, int64_t K, int64_t M, int64_t N
 }  // extern "C"
 
 namespace {
+int recon_row_blocks(const mgx_crt_plan *pl) { return (int)ceil_div(4 * pl->nw, RECON_ROWS); }
+
 // Both reconstruct passes with caller-provided scratch: Wv holds nw x ystride
-// int64 words, gamma ystride int32.
+// int64 words, G the row blocks' gamma shares, nrb x ystride uint64.
 int reconstruct_with(const mgx_crt_plan *pl, int64_t K, int64_t N, const uint32_t *Y, mgx_limb *C, int64_t Crs,
-                     int64_t cap, int64_t *Wv, int32_t *gamma, hipStream_t s)
+                     int64_t cap, int64_t *Wv, uint64_t *G, hipStream_t s)
 {
     const int64_t nent = K * N, ystride = crt_ystride(nent);
-    const int nrb = (int)ceil_div(4 * pl->nw, RECON_ROWS);
+    const int nrb = recon_row_blocks(pl);
     const int64_t total = ceil_div(nent, RECON_ENT) * nrb;
     if (round_up(total, 8) > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "product too large for one reconstruct launch");
     hipLaunchKernelGGL(pl->S == 2 ? crt_recon_gemm_kernel<2> : crt_recon_gemm_kernel<4>,
                        dim3((unsigned)round_up(total, 8)), dim3(256), 0, s, pl->d_pc, (int)pl->P, pl->d_mc,
-                       (int)pl->Kr, nrb, (int)pl->nw, Y, ystride, nent, Wv, gamma, total);
+                       (int)pl->Kr, nrb, (int)pl->nw, Y, ystride, nent, Wv, G, total);
     MGX_HIP(hipGetLastError());
     hipLaunchKernelGGL(crt_recon_carry_kernel, dim3((unsigned)ceil_div(nent, CARRY_ENT)), dim3(256), 0, s, Wv,
-                       ystride, (int)pl->nw, gamma, pl->d_m32, nent, N, (uint64_t *)C, Crs, cap + 1, cap,
+                       ystride, (int)pl->nw, nrb, G, pl->d_m32, nent, N, (uint64_t *)C, Crs, cap + 1, cap,
                        device_error_word());
     MGX_HIP(hipGetLastError());
     return MGX_OK;
@@ -484,7 +486,7 @@ int reconstruct_with(const mgx_crt_plan *pl, int64_t K, int64_t N, const uint32_
 
 size_t recon_scratch_bytes(const mgx_crt_plan *pl, int64_t nent)
 {
-    return (size_t)(pl->nw * crt_ystride(nent) * 8 + crt_ystride(nent) * 4);
+    return (size_t)((pl->nw + recon_row_blocks(pl)) * crt_ystride(nent) * 8);
 }
 }  // namespace
 
@@ -499,7 +501,7 @@ int mgx_crt_reconstruct(const mgx_crt_plan *pl, int64_t K, int64_t N, const uint
     char *scr = nullptr;
     MGX_TRY(scratch_alloc((void **)&scr, recon_scratch_bytes(pl, K * N), s));
     int st = reconstruct_with(pl, K, N, Y, C, Crs, cap, (int64_t *)scr,
-                              (int32_t *)(scr + pl->nw * crt_ystride(K * N) * 8), s);
+                              (uint64_t *)(scr + pl->nw * crt_ystride(K * N) * 8), s);
     scratch_free(scr, s);
     return st;
 }
@@ -579,7 +581,7 @@ int mgx_crt_mfma_matmul(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t 
         int8_t *RA = (int8_t *)scr, *RB = (int8_t *)scr + bytesA;
         uint32_t *Y = (uint32_t *)(scr + offY);
         int64_t *Wv = (int64_t *)(scr + offR);
-        int32_t *gam = (int32_t *)(scr + offR + pl->nw * crt_ystride(K * N) * 8);
+        uint64_t *gam = (uint64_t *)(scr + offR + pl->nw * crt_ystride(K * N) * 8);
         st = mgx_crt_reduce(pl, 0, K, M, A, Ars, cap, 0, RA, stream);
         if (st == MGX_OK) st = mgx_crt_reduce(pl, 1, N, M, B, Brs, cap, 0, RB, stream);
         if (st == MGX_OK) st = mgx_crt_residue_gemm(pl, K, M, N, RA, RB, Y, stream);

@@ -37,13 +37,18 @@ __device__ __forceinline__ uint32_t mod_s64(int64_t x, const PrimeConst &c)
 {
     return mod_u64((uint64_t)(x + (int64_t)c.off), c.p, c.mu);   // |x| < 2^62
 }
-// a * b mod p for a, b < p < 2^16: one 32-bit Barrett step with
-// mu32 = floor(2^32 / p) = mu >> 32 leaves r < 2p
-__device__ __forceinline__ uint32_t mulmod16(uint32_t a, uint32_t b, uint32_t p, uint32_t mu32)
+// u mod p for p < 2^16 and any u < 2^32: with mu32 = floor(2^32 / p)
+// (= mu >> 32) the Barrett quotient is short by at most 2, so r < 3p
+__device__ __forceinline__ uint32_t bred32(uint32_t u, uint32_t p, uint32_t mu32)
 {
-    const uint32_t u = a * b;
-    const uint32_t r = u - __umulhi(u, mu32) * p;
-    return r >= p ? r - p : r;
+    uint32_t r = u - __umulhi(u, mu32) * p;
+    r = min(r, r - p);   // r - p wraps above r when r < p
+    return min(r, r - p);
+}
+// a mod p for signed |a| < 2^31 - 2^17: off32 is a multiple of p in (2^31, 2^31 + p]
+__device__ __forceinline__ uint32_t smod32(int32_t a, uint32_t p, uint32_t mu32, uint32_t off32)
+{
+    return bred32((uint32_t)a + off32, p, mu32);
 }
 // four balanced base-256 digits of |r| < 2^30, byte s = digit s
 __device__ __forceinline__ uint32_t slice4(int32_t r)
@@ -449,10 +454,11 @@ __device__ __forceinline__ void gemm_tile(int64_t v,
 //     whole 32-bit words (prime 2w low, 2w+1 high).
 //
 //     Stage layout: [A | B][slice][row 0..127][4 chunks of 16 bytes], the
-//     chunk of K bytes 16c..16c+15 of row r at slot c ^ ((r >> 2) & 3), so
-//     that 8 consecutive rows read at one K chunk hit distinct banks.  One
-//     DMA instruction of a wave writes 1 KB (16 rows) at base + 16*lane;
-//     lane l takes row l/4, slot l%4, i.e. chunk (l & 3) ^ (l >> 4).
+//     chunk of K bytes 16c..16c+15 of row r at slot c ^ g(r) with
+//     g(r) = ((r >> 1) ^ (r >> 3)) & 3, so that 8 (and 16) consecutive rows
+//     read at one K chunk hit distinct banks with 32 or 64 banks.  One DMA
+//     instruction of a wave writes 1 KB (16 rows) at base + 16*lane; lane l
+//     takes row 16*rb16 + l/4, slot l%4, i.e. chunk (l & 3) ^ g(row).
 // ------------------------------------------------------------------------
 constexpr int KCHUNK2 = 32768;   // 2 pairs * 32768 * 2^14 = 2^30 < 2^31
 constexpr int G2_STAGES = 4;
@@ -462,6 +468,7 @@ constexpr int G2_STAGE = 2 * G2_OPERAND;
 typedef __attribute__((address_space(3))) void lds_void;
 
 __device__ __forceinline__ uint32_t lds_offset(const void *p) { return (uint32_t)(uintptr_t)(const lds_void *)p; }
+__device__ __forceinline__ int g2_swz(int r) { return ((r >> 1) ^ (r >> 3)) & 3; }
 
 // 16 bytes from g (per lane) to LDS at base + 16 * lane; base is wave-uniform.
 // Issued as inline assembly so that the compiler does not wait on it: the
@@ -496,7 +503,6 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
     uint32_t *Yw = Y + (int64_t)pair * ystride;
     const int nsteps = (int)(kpad / 64);
     const uint32_t ls0 = __builtin_amdgcn_readfirstlane(lds_offset(LS));
-    const int dchunk = (lane & 3) ^ (lane >> 4);   // K chunk this lane fetches
 
     uint32_t hold[2][2][8];   // digits of prime 2w, two 16-bit values per register
 
@@ -504,7 +510,7 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
         const int q = 2 * pair + half;
         if (q >= P) break;
         const PrimeConst c = pc[q];
-        const uint32_t mu32 = (uint32_t)(c.mu >> 32);
+        const uint32_t mu32 = (uint32_t)(c.mu >> 32), off32 = ((1u << 31) / c.p + 1) * c.p;
         const int8_t *gA = RA + (int64_t)q * 2 * planeA;
         const int8_t *gB = RB + (int64_t)q * 2 * planeB;
         // this wave's 8 DMA sources per stage: u -> operand u>>2 ... see ISSUE
@@ -512,7 +518,7 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             const int idx = wave * 8 + u, op = idx >> 4, s = (idx >> 3) & 1, rb16 = idx & 7;
-            const int64_t row = rb16 * 16 + (lane >> 2);
+            const int row = rb16 * 16 + (lane >> 2), dchunk = (lane & 3) ^ g2_swz(row);
             src[u] = op ? gB + s * planeB + (j0 + row) * kpad + 16 * dchunk
                         : gA + s * planeA + (i0 + row) * kpad + 16 * dchunk;
         }
@@ -532,9 +538,10 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
 #pragma unroll
                 for (int ct = 0; ct < 2; ct++) acc[d][rt][ct] = (v16i){0};
 
-        // x = sum_d acc_d 2^(8d) mod p (|x| < 2^47), plus the running residue
-        // of earlier chunks; the last fold turns it into y = x (M/p)^-1 mod p,
-        // centred, as two balanced digits
+        // x = acc0 + 256 (acc1 + 256 acc2) mod p plus the running residue of
+        // earlier chunks, in 32-bit Barrett steps (|acc_d| <= 2^30 within a
+        // chunk); the last fold turns it into y = x (M/p)^-1 mod p, centred,
+        // as two balanced digits
 #define FOLD2(first_, last_)                                                                         \
         _Pragma("unroll") for (int rt = 0; rt < 2; rt++)                                             \
         _Pragma("unroll") for (int ct = 0; ct < 2; ct++)                                             \
@@ -543,11 +550,11 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
             const int64_t j = j0 + wj * 64 + ct * 32 + l32;                                          \
             uint16_t *yh = (uint16_t *)(Yw + i * colsC + j) + half;                                  \
             const bool in = i < rowsC && j < colsC;                                                  \
-            int64_t x = (first_ || !in) ? 0 : (int64_t)*yh;                                          \
-            x += (int64_t)acc[0][rt][ct][r] + ((int64_t)acc[1][rt][ct][r] << 8) +                   \
-                 ((int64_t)acc[2][rt][ct][r] << 16);                                                 \
-            uint32_t res = mod_s64(x, c);                                                            \
-            if (last_) res = slice2(centre(mulmod16(res, c.inv, c.p, mu32), c.p));                   \
+            const int32_t prev = (first_ || !in) ? 0 : (int32_t)*yh;                                 \
+            const uint32_t r2 = smod32(acc[2][rt][ct][r], c.p, mu32, off32);                        \
+            const uint32_t z = smod32(acc[1][rt][ct][r] + 256 * (int32_t)r2, c.p, mu32, off32);     \
+            uint32_t res = smod32(acc[0][rt][ct][r] + 256 * (int32_t)z + prev, c.p, mu32, off32);   \
+            if (last_) res = slice2(centre(bred32(res * c.inv, c.p, mu32), c.p));                    \
             if (!CHUNKED && last_) {                                                                 \
                 if (half == 0) {                                                                     \
                     if (r & 1) hold[rt][ct][r >> 1] |= res << 16;                                    \
@@ -589,8 +596,8 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
 #pragma unroll
                     for (int x = 0; x < 2; x++) {
                         const int ra = wi * 64 + x * 32 + l32, rb = wj * 64 + x * 32 + l32;
-                        af[s][x] = *(const v4i *)(sa + (s * 128 + ra) * 64 + 16 * (ch ^ ((ra >> 2) & 3)));
-                        bf[s][x] = *(const v4i *)(sb + (s * 128 + rb) * 64 + 16 * (ch ^ ((rb >> 2) & 3)));
+                        af[s][x] = *(const v4i *)(sa + (s * 128 + ra) * 64 + 16 * (ch ^ g2_swz(ra)));
+                        bf[s][x] = *(const v4i *)(sb + (s * 128 + rb) * 64 + 16 * (ch ^ g2_swz(rb)));
                     }
 #pragma unroll
                 for (int s = 0; s < 2; s++)
@@ -644,9 +651,10 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
 //        byte positions) against the digits of yc_q (columns = entries).  The
 //        epilogue folds each group of four byte rows into one signed 32-bit
 //        word value (|value| < 2^49) stored word-major, Wv[t][e].  The
-//        workgroups of the first row block also form the 32-bit fixed-point
-//        sum of y_q / p_q and the count of negative yc_q into
-//        gamma[e] = round(sum) - count, so that C = X' - gamma M.
+//        workgroups also form the 32-bit fixed-point sum of y_q / p_q and the
+//        count of negative yc_q (gamma = round(sum) - count, so that
+//        C = X' - gamma M): row block rb takes the K steps ks = rb mod nrb
+//        and writes its share as sum + (count << 50) to G[rb][e].
 //    (b) recon_carry_block: one thread per entry finds the sign of
 //        X' - gamma M from its top words down, then runs the carry chain of
 //        |X' - gamma M| low to high; limbs leave through LDS so that each
@@ -658,13 +666,12 @@ constexpr int RECON_ENT = 256;    // entries per workgroup (64 per wave)
 template <int S>
 __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *__restrict__ pc, int P,
     const int8_t *__restrict__ Mc, int Kr, int nrb, int nw, const uint32_t *__restrict__ Y, int64_t ystride,
-    int64_t nent, int64_t *__restrict__ Wv, int32_t *__restrict__ gamma)
+    int64_t nent, int64_t *__restrict__ Wv, uint64_t *__restrict__ G)
 {
     __shared__ __attribute__((aligned(16))) int8_t LA[RECON_ROWS * LDS_PITCH];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, h = lane >> 5, c32 = lane & 31;
     const int rb = (int)(blk % nrb);
     const int64_t e0 = (blk / nrb) * RECON_ENT;
-    const bool first = rb == 0;
     const uint32_t *yp = Y + e0 + 64 * wave + c32;
 
     v16i acc[4][2];
@@ -705,7 +712,7 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
         __syncthreads();                                                                             \
         if (k0 + 64 < Kr) { ALOAD(pa, k0 + 64); }                                                    \
         if (k0 + 128 < Kr) { YLOAD(ynext2, k0 + 128); }                                              \
-        if (first) {                                                                                 \
+        if ((k0 / 64) % nrb == rb) {   /* this row block's share of the gamma sums */               \
             _Pragma("unroll") for (int kk = 0; kk < 2; kk++)                                         \
             _Pragma("unroll") for (int i = 0; i < 4; i++)                                            \
             _Pragma("unroll") for (int sub = 0; sub < 4 / S; sub++) {                                \
@@ -756,15 +763,13 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
                 if (t < nw && e < nent) Wv[(int64_t)t * ystride + e] = v;
             }
         }
-    if (first) {
 #pragma unroll
-        for (int ct = 0; ct < 2; ct++) {
-            const uint32_t lo = __shfl_xor((unsigned)gfix[ct], 32), hi = __shfl_xor((unsigned)(gfix[ct] >> 32), 32);
-            const uint64_t f = gfix[ct] + ((uint64_t)hi << 32 | lo);
-            const int cnt = gcnt[ct] + __shfl_xor(gcnt[ct], 32);
-            const int64_t e = e0 + 64 * wave + 32 * ct + c32;
-            if (h == 0 && e < nent) gamma[e] = (int32_t)((f + (1ull << 31)) >> 32) - cnt;
-        }
+    for (int ct = 0; ct < 2; ct++) {
+        const uint32_t lo = __shfl_xor((unsigned)gfix[ct], 32), hi = __shfl_xor((unsigned)(gfix[ct] >> 32), 32);
+        const uint64_t f = gfix[ct] + ((uint64_t)hi << 32 | lo);   // < P 2^32 <= 2^50
+        const uint64_t cnt = (uint64_t)(gcnt[ct] + __shfl_xor(gcnt[ct], 32));
+        const int64_t e = e0 + 64 * wave + 32 * ct + c32;
+        if (h == 0 && e < nent) G[(int64_t)rb * ystride + e] = f + (cnt << 50);
     }
 }
 
@@ -772,7 +777,7 @@ constexpr int CARRY_ENT = 256;   // entries per workgroup
 constexpr int CARRY_PITCH = 17;  // limbs per LDS row (16 + 1 padding)
 
 __device__ __forceinline__ void recon_carry_block(int64_t blk, const int64_t *__restrict__ Wv, int64_t ystride,
-    int nw, const int32_t *__restrict__ gamma, const uint32_t *__restrict__ M32, int64_t nent, int64_t colsC,
+    int nw, int nrb, const uint64_t *__restrict__ G, const uint32_t *__restrict__ M32, int64_t nent, int64_t colsC,
     uint64_t *__restrict__ C, int64_t Crs, int64_t W, int64_t cap, unsigned *__restrict__ err)
 {
     __shared__ uint64_t LS[CARRY_ENT * CARRY_PITCH];
@@ -781,7 +786,16 @@ __device__ __forceinline__ void recon_carry_block(int64_t blk, const int64_t *__
     const int64_t e = blk * CARRY_ENT + tid;
     const bool valid = e < nent;
     s_base[tid] = valid ? ((e / colsC) * Crs + e % colsC) * W : -1;
-    const int64_t gam = valid ? gamma[e] : 0;
+    int64_t gam = 0;
+    if (valid) {   // gamma from the row blocks' shares
+        uint64_t f = 0, cnt = 0;
+        for (int rb = 0; rb < nrb; rb++) {
+            const uint64_t g = G[(int64_t)rb * ystride + e];
+            f += g & ((1ull << 50) - 1);
+            cnt += g >> 50;
+        }
+        gam = (int64_t)((f + (1ull << 31)) >> 32) - (int64_t)cnt;
+    }
     const int64_t *wp = Wv + (valid ? e : 0);
 
     // Sign: x_t = Wv[t] - gam M32[t] is below 2^56 in magnitude (int32 column

@@ -266,6 +266,39 @@ int mgx_costmodel_set_normalisation(mgx_costmodel *cm, const float *mean, const 
     return MGX_OK;
 }
 
+// Text form written by costmodel.py: "mgx-costmodel 1", then D H O, then D
+// means, D scales and the packed parameters, whitespace separated.
+int mgx_costmodel_load(mgx_costmodel **out, const char *path)
+{
+    if (!out || !path) return fail(MGX_ERR_ARG, "null argument");
+    FILE *f = fopen(path, "r");
+    if (!f) return fail(MGX_ERR_ARG, "cannot open cost model %s", path);
+    char tag[32] = {0};
+    int version = 0, D = 0, H = 0, O = 0;
+    int st = MGX_OK;
+    if (fscanf(f, "%31s %d %d %d %d", tag, &version, &D, &H, &O) != 5 || strcmp(tag, "mgx-costmodel") != 0 ||
+        version != 1 || D < 1 || H < 1 || O < 1 || D > 4096 || H > 4096 || O > 4096)
+        st = fail(MGX_ERR_PARSE, "%s is not a cost model file", path);
+    mgx_costmodel *cm = nullptr;
+    if (st == MGX_OK) st = mgx_costmodel_create(&cm, D, H, O, 0);
+    if (st == MGX_OK) {
+        std::vector<float> mean(D), scale(D), p(cm->np);
+        bool ok = true;
+        for (auto *v : {&mean, &scale, &p})
+            for (float &x : *v) ok = ok && fscanf(f, "%f", &x) == 1;
+        if (!ok) st = fail(MGX_ERR_PARSE, "%s: truncated cost model", path);
+        if (st == MGX_OK) st = mgx_costmodel_set_params(cm, p.data());
+        if (st == MGX_OK) st = mgx_costmodel_set_normalisation(cm, mean.data(), scale.data());
+        if (st != MGX_OK) {
+            mgx_costmodel_destroy(cm);
+            cm = nullptr;
+        }
+    }
+    fclose(f);
+    if (st == MGX_OK) *out = cm;
+    return st;
+}
+
 // Loss and gradient (into grad_out, host) for a batch; X is raw (normalised here).
 int mgx_costmodel_loss_grad(mgx_costmodel *cm, const float *X, const float *T, const float *mask, int Bn,
                             float *grad_out, float *loss_out)

@@ -8,6 +8,7 @@
 // mat-gpu/mat-gpu.cu:20,389).  The public matrix-level calls synchronize once
 // at the end and report the device error word as a status.
 #include "common.h"
+#include "../../include/mgx_sched.h"
 #include <chrono>
 #include <algorithm>
 
@@ -367,6 +368,7 @@ int mgx_matmul_host(int method, int64_t K, int64_t M, int64_t N, mgx_limb *C, co
             break;
         }
         case 8: st = mgx_crt_mfma_matmul(nullptr, K, M, N, dC, N, dA, M, dB, N, cap, s); break;
+        case 0: st = mgx_matmul_route(MGX_ROUTE_AUTO, K, M, N, dC, N, dA, M, dB, N, cap, s); break;
         default: st = fail(MGX_ERR_ARG, "invalid method %d", method);
         }
     }

@@ -106,6 +106,32 @@ def _declare(L):
     L.mgx_crt_plan_slices.restype = c_i64
 
 
+class Product(ctypes.Structure):
+    """mgx_product (include/mgx_sched.h): one product of a batch."""
+    _fields_ = [("K", c_i64), ("M", c_i64), ("N", c_i64), ("C", c_vp), ("Crs", c_i64), ("A", c_vp), ("Ars", c_i64),
+                ("B", c_vp), ("Brs", c_i64), ("cap", c_i64), ("bitsA", c_i64), ("bitsB", c_i64),
+                ("route", ctypes.c_int)]
+
+
+def bind_sched():
+    """Argument types of the batch, dispatch and device-generator entry points."""
+    L = lib()
+    L.mgx_batch_create.argtypes = [ctypes.POINTER(c_vp), c_vp, c_i64, c_vp]
+    L.mgx_batch_run.argtypes = [c_vp, c_vp]
+    L.mgx_batch_route.argtypes = [c_vp, c_i64]
+    L.mgx_batch_scratch_bytes.argtypes = [c_vp]
+    L.mgx_batch_scratch_bytes.restype = c_i64
+    L.mgx_batch_destroy.argtypes = [c_vp]
+    L.mgx_random_matrix_device.argtypes = [c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_i64, ctypes.c_double,
+                                           ctypes.c_uint64, c_vp]
+    L.mgx_matmul_route.argtypes = [ctypes.c_int, c_i64, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i64,
+                                   c_vp]
+    L.mgx_costmodel_load.argtypes = [ctypes.POINTER(c_vp), ctypes.c_char_p]
+    L.mgx_costmodel_destroy.argtypes = [c_vp]
+    L.mgx_dispatch_set_model.argtypes = [c_vp]
+    return L
+
+
 def check(status):
     if status != 0:
         raise MgxError(status, lib().mgx_last_error().decode())

@@ -6,6 +6,13 @@ sampled entries.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--plain-steps P]
                   [--n 1024] [--bits 2048] [--cpu-seconds S]
+  python bench.py --batch [--products 512] [--chunk 32] [--steps K]
+
+--batch times the mixed batch instead (one more JSON line): 512 independent
+exact products, n uniform in [16, 512], entry bits log-uniform in [64, 16384],
+10% rectangular (K x 2K x K), routed by the committed cost model and run as
+hipGraph replays of the persistent-kernel work queues, chunk products per
+graph; every 8th product is compared with the single-product CRT route.
 """
 import argparse
 import ctypes
@@ -40,10 +47,15 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", action="store_true")
+    ap.add_argument("--products", type=int, default=512)
+    ap.add_argument("--chunk", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.batch:
+        return run_batch(args)
 
     import mgx
     mgx.init(0)
@@ -202,6 +214,95 @@ def main():
     }
     print(json.dumps(line), flush=True)
     mgx.call("mgx_crt_plan_destroy", plan)
+
+
+def run_batch(args):
+    import mgx
+    mgx.init(0)
+    L = mgx.bind_sched()
+    rng = np.random.default_rng(20240)
+    specs = []
+    for i in range(args.products):
+        n = int(rng.integers(16, 513))
+        bits = int(round(2 ** rng.uniform(6, 14)))
+        rect = rng.random() < 0.1
+        K, M, N = n, (2 * n if rect else n), n
+        cap = (2 * bits + int(np.ceil(np.log2(M))) + 2 + 63) // 64 + 1
+        specs.append((K, M, N, bits, cap))
+    model = mgx.c_vp()
+    mpath = os.path.join(ROOT, "bachelor-thesis_amd", "costmodel_params.txt")
+    mgx.check(L.mgx_costmodel_load(ctypes.byref(model), mpath.encode()))
+    mgx.check(L.mgx_dispatch_set_model(model))
+    stream = mgx.c_vp()
+    mgx.call("mgx_stream_create", ctypes.byref(stream))
+    s = stream.value
+    total_s, routes, checked, exact = 0.0, [0, 0, 0, 0], 0, True
+    entry_products = sum(K * M * N for K, M, N, _, _ in specs)
+    for c0 in range(0, len(specs), args.chunk):
+        chunk = specs[c0:c0 + args.chunk]
+        bufs = []
+        prods = (mgx.Product * len(chunk))()
+        for j, (K, M, N, bits, cap) in enumerate(chunk):
+            W = cap + 1
+            dA, dB, dC = mgx.DeviceBuffer(K * M * W * 8), mgx.DeviceBuffer(M * N * W * 8), mgx.DeviceBuffer(K * N * W * 8)
+            seed = 1000 + 2 * (c0 + j)
+            mgx.check(L.mgx_random_matrix_device(dA.ptr, M, K, M, cap, bits, bits, 0.0, seed, s))
+            mgx.check(L.mgx_random_matrix_device(dB.ptr, N, M, N, cap, bits, bits, 0.0, seed + 1, s))
+            bufs.append((dA, dB, dC))
+            prods[j] = mgx.Product(K, M, N, dC.ptr, N, dA.ptr, M, dB.ptr, N, cap, bits, bits, -1)
+        b = mgx.c_vp()
+        mgx.check(L.mgx_batch_create(ctypes.byref(b), prods, len(chunk), s))
+        for j in range(len(chunk)):
+            routes[L.mgx_batch_route(b, j)] += 1
+        mgx.check(L.mgx_batch_run(b, s))          # warm-up replay
+        mgx.call("mgx_check_error", s)
+        mgx.call("mgx_stream_sync", s)
+        steps = max(args.steps, 1)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            mgx.check(L.mgx_batch_run(b, s))
+        mgx.call("mgx_stream_sync", s)
+        total_s += (time.perf_counter() - t0) / steps
+        mgx.call("mgx_check_error", s)
+        for j in range(0, len(chunk), 8):   # the batch against the single-product CRT route
+            K, M, N, bits, cap = chunk[j]
+            dA, dB, dC = bufs[j]
+            ref = mgx.DeviceBuffer(K * N * (cap + 1) * 8)
+            mgx.call("mgx_crt_mfma_matmul", None, K, M, N, ref.ptr, N, dA.ptr, M, dB.ptr, N, cap, s)
+            mgx.call("mgx_stream_sync", s)
+            x, y = dC.to_array((K, N, cap + 1)), ref.to_array((K, N, cap + 1))
+            sz = x[..., cap].view(np.int64)
+            ok = bool((sz == y[..., cap].view(np.int64)).all())
+            if ok:
+                mask = np.arange(cap)[None, None, :] < np.abs(sz)[..., None]
+                ok = bool((np.where(mask, x[..., :cap], 0) == np.where(mask, y[..., :cap], 0)).all())
+            exact = exact and ok
+            checked += 1
+            ref.free()
+        scratch = L.mgx_batch_scratch_bytes(b)
+        L.mgx_batch_destroy(b)
+        for t in bufs:
+            for d in t:
+                d.free()
+        log("products %d..%d: %.3f ms per replay, scratch %.2f GB" % (c0, c0 + len(chunk) - 1, total_s * 1e3,
+                                                                      scratch / 1e9))
+    L.mgx_dispatch_set_model(None)
+    L.mgx_costmodel_destroy(model)
+    line = {
+        "metric": "mixed batch: %d exact products wall-s" % len(specs),
+        "value": total_s, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": total_s * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int8 MFMA / int64",
+        "data": "synthetic: seeded xoshiro256** entries per product, uniform magnitudes, random signs",
+        "config": {"workload": "%d products, n~U[16,512], bits log-uniform [64,16384], 10%% K x 2K x K; "
+                               "learned dispatch; hipGraph replays of %d-product work-queue batches"
+                               % (len(specs), args.chunk)},
+        "products_per_s": len(specs) / total_s,
+        "entry_products_per_s": entry_products / total_s,
+        "routes": {"crt": routes[0], "plain": routes[1] + routes[2] + routes[3]},
+        "bit_exact": {"vs_single_product_crt": exact, "products_checked": checked},
+    }
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

@@ -66,6 +66,9 @@ int mgx_costmodel_loss_grad(mgx_costmodel *cm, const float *X, const float *T, c
 int mgx_costmodel_train(mgx_costmodel *cm, const float *X, const float *T, const float *mask, int B, int epochs,
                         float lr, float *loss);
 int mgx_costmodel_predict(mgx_costmodel *cm, const float *X, int B, float *Y);
+/* Load a trained model from the text file costmodel.py writes
+ * ("mgx-costmodel 1 D H O", means, scales, packed parameters). */
+int mgx_costmodel_load(mgx_costmodel **cm, const char *path);
 /* Installs the model used by MGX_ROUTE_AUTO (NULL: none; the autotuner's
  * exhaustive timing decides instead).  The library does not take ownership. */
 int mgx_dispatch_set_model(mgx_costmodel *cm);

@@ -38,6 +38,16 @@ def test_generated_product_and_json():
     assert r2.stdout.strip().split("\n")[:3] == lines[:3]
 
 
+def test_cli_learned_dispatch():
+    # -m 0: the route the committed cost model picks, same product as -m 8
+    args = ["-n", "5,7,6", "-b", "700", "-s", "4", "-p"]
+    r0 = subprocess.run([BIN] + args + ["-m", "0"], capture_output=True, text=True, timeout=120)
+    r8 = subprocess.run([BIN] + args + ["-m", "8"], capture_output=True, text=True, timeout=120)
+    assert r0.returncode == 0 and r8.returncode == 0, r0.stderr + r8.stderr
+    assert "No cost model" not in r0.stderr
+    assert r0.stdout == r8.stdout
+
+
 def test_cli_errors():
     r = subprocess.run([BIN, "-n", "2,2,2"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "No bit count" in r.stderr

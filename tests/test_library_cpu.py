@@ -89,3 +89,21 @@ def test_hex_roundtrip_matches_reference_format():
         assert entry_int(back) == v
     bad = np.zeros(cap + 1, dtype=np.uint64)
     assert L.mgx_entry_from_hex(bad.ctypes.data, cap, b"xyz") == 7
+
+
+def test_committed_cost_model_files_agree():
+    """The text model (read by mgx_costmodel_load) and the JSON copy hold the
+    same 3-layer, 64-hidden network."""
+    import json
+    here = os.path.join(ROOT, "bachelor-thesis_amd")
+    with open(os.path.join(here, "costmodel_params.json")) as f:
+        d = json.load(f)
+    with open(os.path.join(here, "costmodel_params.txt")) as f:
+        head = f.readline().split()
+        rows = [np.array(line.split(), dtype=np.float64) for line in f]
+    assert head[:2] == ["mgx-costmodel", "1"]
+    D, H, O = (int(x) for x in head[2:5])
+    assert (D, H, O) == (d["D"], d["H"], d["O"]) == (13, 64, 4)
+    assert len(rows) == 3 and rows[0].size == D and rows[1].size == D
+    assert rows[2].size == H * D + H + H * H + H + O * H + O
+    assert np.allclose(rows[2], d["params"], rtol=1e-6) and np.allclose(rows[0], d["mean"], rtol=1e-6)

@@ -189,3 +189,46 @@ def test_tuner_and_dispatch(gpu):
     f = np.zeros(13, dtype=np.float32)
     mgx.check(L.mgx_tune_features_device(K, M, N, dA.ptr, M, dB.ptr, N, cap, f.ctypes.data))
     assert f[0] == 5 and f[5] == pytest.approx(8.0, abs=0.01) and abs(f[7:].sum() - 1) < 1e-6
+
+
+def test_committed_model_loads_and_dispatches(gpu):
+    """The trained model file: the C loader gives the same predictions as the
+    parameters set from the JSON copy, and MGX_ROUTE_AUTO with it installed is
+    exact."""
+    import json
+    import os
+    mgx = gpu
+    L = sched(mgx)
+    L.mgx_costmodel_load.argtypes = [ctypes.POINTER(vp), ctypes.c_char_p]
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "bachelor-thesis_amd")
+    cm = vp()
+    mgx.check(L.mgx_costmodel_load(ctypes.byref(cm), os.path.join(here, "costmodel_params.txt").encode()))
+    with open(os.path.join(here, "costmodel_params.json")) as f:
+        d = json.load(f)
+    cm2 = vp()
+    mgx.check(L.mgx_costmodel_create(ctypes.byref(cm2), d["D"], d["H"], d["O"], 0))
+    p = np.array(d["params"], dtype=np.float32)
+    mean, scale = np.array(d["mean"], dtype=np.float32), np.array(d["scale"], dtype=np.float32)
+    mgx.check(L.mgx_costmodel_set_params(cm2, p.ctypes.data))
+    mgx.check(L.mgx_costmodel_set_normalisation(cm2, mean.ctypes.data, scale.ctypes.data))
+    X = (mean + scale * np.random.default_rng(3).standard_normal((17, d["D"]))).astype(np.float32)
+    Y1, Y2 = np.zeros((17, d["O"]), dtype=np.float32), np.zeros((17, d["O"]), dtype=np.float32)
+    mgx.check(L.mgx_costmodel_predict(cm, X.ctypes.data, 17, Y1.ctypes.data))
+    mgx.check(L.mgx_costmodel_predict(cm2, X.ctypes.data, 17, Y2.ctypes.data))
+    assert np.allclose(Y1, Y2, rtol=1e-6, atol=1e-5)
+    try:
+        mgx.check(L.mgx_dispatch_set_model(cm))
+        K, M, N, bits, cap = 19, 23, 17, 900, 32
+        A, B = rand_ints(K, M, bits, 41), rand_ints(M, N, bits, 42)
+        dA, dB = mgx.DeviceBuffer.from_array(to_entries(A, cap)), mgx.DeviceBuffer.from_array(to_entries(B, cap))
+        dC = mgx.DeviceBuffer(K * N * (cap + 1) * 8)
+        r = ctypes.c_int()
+        mgx.check(L.mgx_dispatch_choose(K, M, N, dA.ptr, M, dB.ptr, N, cap, ctypes.byref(r)))
+        assert 0 <= r.value < 4
+        mgx.check(L.mgx_matmul_route(-1, K, M, N, dC.ptr, N, dA.ptr, M, dB.ptr, N, cap, None))
+        C = dC.to_array((K, N, cap + 1))
+        assert to_ints(C) == matmul(A, B) and is_normalized(C)
+    finally:
+        L.mgx_dispatch_set_model(None)
+        L.mgx_costmodel_destroy(cm)
+        L.mgx_costmodel_destroy(cm2)
