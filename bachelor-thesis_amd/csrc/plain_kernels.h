@@ -159,16 +159,18 @@ __device__ __forceinline__ void columns_to_digits(const Acc96 (&acc)[G], uint32_
 
 // --------------------------------------------------------- mul / addmul
 // Block product a (this lane's word a_l) x b: column l (lo) and l+64 (hi).
+// Only the first cnt words of a can be non-zero (cnt wave-uniform, 1..64),
+// so the rotation stops after cnt steps.
 struct ColPair {
     Acc96 lo, hi;
 };
-__device__ __noinline__ ColPair block_product_fn(uint32_t a, uint32_t b)
+__device__ __noinline__ ColPair block_product_fn(uint32_t a, uint32_t b, int cnt)
 {
     const uint32_t l = lane_id();
     uint32_t br = b;
     ColPair r = {{0, 0}, {0, 0}};
 #pragma unroll 8
-    for (int i = 0; i < 64; i++) {
+    for (int i = 0; i < cnt; i++) {
         uint32_t ai = rdlane(a, i);
         uint64_t p = (uint64_t)ai * br;
         if (l >= (uint32_t)i) add96(r.lo, p);
@@ -177,11 +179,18 @@ __device__ __noinline__ ColPair block_product_fn(uint32_t a, uint32_t b)
     }
     return r;
 }
-__device__ __forceinline__ void block_product(uint32_t a, uint32_t b, Acc96 &lo, Acc96 &hi)
+// The product is symmetric, so the block with fewer valid words is the one
+// iterated: a 65-word Winograd sum times a 64-word entry costs 64 + 1 steps.
+__device__ __forceinline__ void block_product(uint32_t a, uint32_t b, int cnta, int cntb, Acc96 &lo, Acc96 &hi)
 {
-    ColPair r = block_product_fn(a, b);
+    ColPair r = cnta <= cntb ? block_product_fn(a, b, cnta) : block_product_fn(b, a, cntb);
     add96(lo, r.lo);
     add96(hi, r.hi);
+}
+__device__ __forceinline__ int block_words(int64_t n, int blk)
+{
+    const int64_t c = n - 64 * (int64_t)blk;
+    return c <= 0 ? 0 : c >= 64 ? 64 : (int)c;
 }
 
 template <int G>
@@ -277,7 +286,9 @@ __device__ __forceinline__ void plain_entries(int64_t blk, int64_t K, int64_t M,
         }
         const bool neg = (sa < 0) != (sb < 0);
         const int nbm = nba > nbb ? nba : nbb;
-        if (toom && nbm >= 2 && (nbm & 1) == 0 && nbm >= toom && nba == nbb) {
+        // Karatsuba pays only when both top halves are nearly full blocks
+        if (toom && nbm >= 2 && (nbm & 1) == 0 && nbm >= toom && nba == nbb && na >= 64 * nbm - 16 &&
+            nb >= 64 * nbm - 16) {
             // Karatsuba: a b = a0 b0 (1 + X) + a1 b1 (X + X^2) - (a0 - a1)(b0 - b1) X, X = 2^(32*64*h)
             const int h = nbm / 2;
             uint32_t da[NB], db[NB];
@@ -293,9 +304,9 @@ __device__ __forceinline__ void plain_entries(int64_t blk, int64_t K, int64_t M,
                     const uint32_t a0 = a[ib], b0 = b[jb], am = da[ib], bm = db[jb];
                     const uint32_t a1 = sel<NB>(a, ib + h), b1 = sel<NB>(b, jb + h);
                     Acc96 z0l = {0, 0}, z0h = {0, 0}, z2l = {0, 0}, z2h = {0, 0}, zml = {0, 0}, zmh = {0, 0};
-                    block_product(a0, b0, z0l, z0h);
-                    block_product(a1, b1, z2l, z2h);
-                    block_product(am, bm, zml, zmh);
+                    block_product(a0, b0, 64, 64, z0l, z0h);
+                    block_product(a1, b1, block_words(na, ib + h), block_words(nb, jb + h), z2l, z2h);
+                    block_product(am, bm, 64, 64, zml, zmh);
                     const int g0 = ib + jb;
                     Acc96 sl = z0l, sh = z0h;   // z0 + z2 lands at offset h
                     add96(sl, z2l);
@@ -319,7 +330,7 @@ __device__ __forceinline__ void plain_entries(int64_t blk, int64_t K, int64_t M,
                 for (int jb = 0; jb < NB; jb++) {
                     if (ib >= nba || jb >= nbb) continue;
                     Acc96 zl = {0, 0}, zh = {0, 0};
-                    block_product(a[ib], b[jb], zl, zh);
+                    block_product(a[ib], b[jb], block_words(na, ib), block_words(nb, jb), zl, zh);
                     if (neg) { add96(accN[ib + jb], zl); add96(accN[ib + jb + 1], zh); }
                     else { add96(accP[ib + jb], zl); add96(accP[ib + jb + 1], zh); }
                 }

@@ -36,6 +36,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def pmc_traffic(kernel, n, bits):
+    """HBM bytes per dispatch of `kernel` at this config, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated against a plain copy);
+    None when no summary covers it."""
+    import glob
+    import re
+    pat = re.compile(r"^traffic_bytes %s n=%d bits=%d: (\d+)$" % (re.escape(kernel), n, bits))
+    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(f) as fh:
+            for line in fh:
+                m = pat.match(line.strip())
+                if m:
+                    return int(m.group(1)), os.path.relpath(f, ROOT)
+    return None, None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +139,8 @@ def main():
     log("stages ms: reduceA %.3f reduceB %.3f gemm %.3f reconstruct %.3f (P=%d)" % tuple(stage_ms + [P]))
     gemm_ops = 2.0 * n ** 3 * S * S * P
     gemm_tops = gemm_ops / (stage_ms[2] * 1e-3) / 1e12
+    gemm_kernel = "crt_gemm2_kernel" if S == 2 else "crt_gemm_kernel"
+    traffic, traffic_src = pmc_traffic(gemm_kernel, n, bits)
     C = dC.to_array((n, n, W))
 
     # ---- plain route (Winograd over per-entry mpn multiplication), same inputs
@@ -207,9 +225,10 @@ def main():
         "bit_exact": {"crt_vs_plain_all_entries": bit_exact_routes, "crt_vs_oracle_sample": sample_ok},
         "stage_ms": {"reduce_A": stage_ms[0], "reduce_B": stage_ms[1], "residue_gemm": stage_ms[2],
                      "reconstruct": stage_ms[3]},
-        "roofline": {"bound": "mfma", "kernel": "crt_gemm2_kernel" if S == 2 else "crt_gemm_kernel", "achieved": gemm_tops, "peak": INT8_PEAK_TOPS,
-                     "unit": "TFLOP/s", "frac": gemm_tops / INT8_PEAK_TOPS, "traffic": None,
-                     "note": "int8 ops 2*n^3*S^2*P per launch / event-timed launch duration"},
+        "roofline": {"bound": "mfma", "kernel": gemm_kernel, "achieved": gemm_tops, "peak": INT8_PEAK_TOPS,
+                     "unit": "TFLOP/s", "frac": gemm_tops / INT8_PEAK_TOPS, "traffic": traffic,
+                     "note": "int8 ops 2*n^3*S^2*P per launch / event-timed launch duration; traffic = HBM bytes "
+                             "per launch from %s" % (traffic_src or "no PMC summary for this config")},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
