@@ -46,6 +46,14 @@ int ensure_init()
         MGX_HIP(hipMalloc(&w, 256));
         MGX_HIP(hipMemset(w, 0, 256));
         g_errword[dev] = w;
+        // stream-ordered scratch (residues, Y, column sums: gigabytes per
+        // product) stays mapped in the device's pool between calls instead of
+        // going back to the driver at every synchronization
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+            uint64_t keep = ~0ull;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
     }
     g_init = true;
     return MGX_OK;

@@ -108,14 +108,14 @@ __global__ __launch_bounds__(256, 1) void batch_gemm_kernel(Queue q, const GemmJ
 }
 
 template <bool CHUNKED>
-__global__ __launch_bounds__(256, 1) void batch_gemm2_kernel(Queue q, const GemmJob *jobs)
+__global__ __launch_bounds__(256, 2) void batch_gemm2_kernel(Queue q, const GemmJob *jobs)
 {
     int j;
     int64_t t;
     while (next_item(q, j, t)) {
         const GemmJob &g = jobs[j];
-        gemm_tile2<CHUNKED>(t, g.pc, g.P, g.RA, g.RB, g.planeA, g.planeB, g.kpad, g.rowsC, g.colsC, g.ntm, g.ntn,
-                            g.Y, g.ystride);
+        gemm_tile2<CHUNKED, 2, 2>(t, g.pc, g.P, g.RA, g.RB, g.planeA, g.planeB, g.kpad, g.rowsC, g.colsC, g.ntm,
+                                  g.ntn, g.Y, g.ystride);
         __syncthreads();
     }
 }
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(256, 2) void batch_recon_gemm_kernel(Queue q, const
     int64_t t;
     while (next_item(q, j, t)) {
         const ReconJob &r = jobs[j];
-        recon_gemm_tile<S>(t, r.pc, r.P, r.Mc, r.Kr, r.nrb, r.nw, r.Y, r.ystride, r.nent, r.Wv, r.G);
+        recon_gemm_tile<S, false>(t, r.pc, r.P, r.Mc, r.Kr, r.nrb, r.nw, r.Y, r.ystride, r.nent, r.Wv, r.G);
         __syncthreads();
     }
 }

@@ -100,8 +100,10 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_kernel(
     gemm_tile<CHUNKED>(v, pc, RA, RB, planeA, planeB, kpad, rowsC, colsC, ntm, ntn, total, Y, ystride);
 }
 
-template <bool CHUNKED>
-__global__ __launch_bounds__(256, 1) void crt_gemm2_kernel(
+// Two workgroups of 4 waves share a CU (two LDS stages of 32 KB each), so
+// one workgroup's fold overlaps the other's MFMAs.
+template <bool CHUNKED, int WM, int STAGES>
+__global__ __launch_bounds__(128 * WM, 2) void crt_gemm2_kernel(
     const PrimeConst *__restrict__ pc, int P, const int8_t *__restrict__ RA, const int8_t *__restrict__ RB,
     int64_t planeA, int64_t planeB, int64_t kpad, int64_t rowsC, int64_t colsC, int ntm, int ntn, int64_t total,
     uint32_t *__restrict__ Y, int64_t ystride)
@@ -110,10 +112,10 @@ __global__ __launch_bounds__(256, 1) void crt_gemm2_kernel(
     int64_t per = (total + 7) / 8;
     int64_t v = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
     if (v >= total) return;
-    gemm_tile2<CHUNKED>(v, pc, P, RA, RB, planeA, planeB, kpad, rowsC, colsC, ntm, ntn, Y, ystride);
+    gemm_tile2<CHUNKED, WM, STAGES>(v, pc, P, RA, RB, planeA, planeB, kpad, rowsC, colsC, ntm, ntn, Y, ystride);
 }
 
-template <int S>
+template <int S, bool RESIDENT>
 __global__ __launch_bounds__(256, 2) void crt_recon_gemm_kernel(
     const PrimeConst *__restrict__ pc, int P, const int8_t *__restrict__ Mc, int Kr, int nrb, int nw,
     const uint32_t *__restrict__ Y, int64_t ystride, int64_t nent, int64_t *__restrict__ Wv,
@@ -123,7 +125,7 @@ __global__ __launch_bounds__(256, 2) void crt_recon_gemm_kernel(
     int64_t per = (total + 7) / 8;
     int64_t v = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
     if (v >= total) return;
-    recon_gemm_tile<S>(v, pc, P, Mc, Kr, nrb, nw, Y, ystride, nent, Wv, G);
+    recon_gemm_tile<S, RESIDENT>(v, pc, P, Mc, Kr, nrb, nw, Y, ystride, nent, Wv, G);
 }
 
 __global__ __launch_bounds__(256) void crt_recon_carry_kernel(
@@ -433,14 +435,9 @@ int mgx_crt_residue_gemm(const mgx_crt_plan *pl, int64_t K, int64_t M, int64_t N
         int64_t total = (int64_t)ntm * ntn * ceil_div(pl->P, 2);
         int64_t grid = round_up(total, 8);
         if (grid > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "product too large for one GEMM launch");
-        if (kpad <= KCHUNK2)
-            hipLaunchKernelGGL(crt_gemm2_kernel<false>, dim3((unsigned)grid), dim3(256), 0, as_stream(stream),
-                               pl->d_pc, (int)pl->P, RA, RB, rpadA * kpad, rpadB * kpad, kpad, K, N, ntm, ntn, total,
-                               Y, ystride);
-        else
-            hipLaunchKernelGGL(crt_gemm2_kernel<true>, dim3((unsigned)grid), dim3(256), 0, as_stream(stream),
-                               pl->d_pc, (int)pl->P, RA, RB, rpadA * kpad, rpadB * kpad, kpad, K, N, ntm, ntn, total,
-                               Y, ystride);
+        auto kern = kpad > KCHUNK2 ? crt_gemm2_kernel<true, 2, 2> : crt_gemm2_kernel<false, 2, 2>;
+        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, as_stream(stream), pl->d_pc, (int)pl->P, RA,
+                           RB, rpadA * kpad, rpadB * kpad, kpad, K, N, ntm, ntn, total, Y, ystride);
         MGX_HIP(hipGetLastError());
         return MGX_OK;
     }
@@ -473,9 +470,13 @@ int reconstruct_with(const mgx_crt_plan *pl, int64_t K, int64_t N, const uint32_
     const int nrb = recon_row_blocks(pl);
     const int64_t total = ceil_div(nent, RECON_ENT) * nrb;
     if (round_up(total, 8) > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "product too large for one reconstruct launch");
-    hipLaunchKernelGGL(pl->S == 2 ? crt_recon_gemm_kernel<2> : crt_recon_gemm_kernel<4>,
-                       dim3((unsigned)round_up(total, 8)), dim3(256), 0, s, pl->d_pc, (int)pl->P, pl->d_mc,
-                       (int)pl->Kr, nrb, (int)pl->nw, Y, ystride, nent, Wv, G, total);
+    const int64_t lds = recon_res_lds(pl->Kr);
+    const bool resident = lds <= RECON_RES_MAX_LDS;
+    auto kern = pl->S == 2 ? (resident ? crt_recon_gemm_kernel<2, true> : crt_recon_gemm_kernel<2, false>)
+                           : (resident ? crt_recon_gemm_kernel<4, true> : crt_recon_gemm_kernel<4, false>);
+    if (resident) MGX_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kern, dim3((unsigned)round_up(total, 8)), dim3(256), resident ? (unsigned)lds : 0u, s, pl->d_pc,
+                       (int)pl->P, pl->d_mc, (int)pl->Kr, nrb, (int)pl->nw, Y, ystride, nent, Wv, G, total);
     MGX_HIP(hipGetLastError());
     hipLaunchKernelGGL(crt_recon_carry_kernel, dim3((unsigned)ceil_div(nent, CARRY_ENT)), dim3(256), 0, s, Wv,
                        ystride, (int)pl->nw, nrb, G, pl->d_m32, nent, N, (uint64_t *)C, Crs, cap + 1, cap,

@@ -445,13 +445,15 @@ __device__ __forceinline__ void gemm_tile(int64_t v,
 
 // ------------------------------------------------------------------------
 // 2b. residue GEMM for two slices (primes below 65280).  One workgroup
-//     takes a 128 x 128 tile of a pair of primes 2w, 2w+1 in turn; 4 waves
+//     takes a (64 WM) x 128 tile of a pair of primes 2w, 2w+1 in turn; 2 WM waves
 //     of 64 x 64 on v_mfma_i32_32x32x32_i8, 3 diagonal accumulators
-//     (s + t = 0, 1, 2).  K runs in steps of 64 bytes through a ring of four
-//     LDS stages filled by global_load_lds_dwordx4 (global memory straight
-//     into LDS, no staging registers) three steps ahead, so that an L2 miss
-//     is covered by three steps of MFMA work.  The pair's digits leave as
-//     whole 32-bit words (prime 2w low, 2w+1 high).
+//     (s + t = 0, 1, 2).  K runs in steps of 64 bytes through a ring of LDS
+//     stages filled by global_load_lds_dwordx4 (global memory straight into
+//     LDS, no staging registers) STAGES - 1 steps ahead.  The launches use
+//     WM = 2 with two stages, two workgroups per CU: one workgroup's fold
+//     (VALU) then runs beside the other's MFMAs, which measured faster than
+//     deeper rings or 256-row tiles with one workgroup per CU.  The pair's digits leave as
+//     whole 32-bit words (prime 2w low, 2w+1 high) through LDS.
 //
 //     Stage layout: [A | B][slice][row 0..127][4 chunks of 16 bytes], the
 //     chunk of K bytes 16c..16c+15 of row r at slot c ^ g(r) with
@@ -461,9 +463,14 @@ __device__ __forceinline__ void gemm_tile(int64_t v,
 //     takes row 16*rb16 + l/4, slot l%4, i.e. chunk (l & 3) ^ g(row).
 // ------------------------------------------------------------------------
 constexpr int KCHUNK2 = 32768;   // 2 pairs * 32768 * 2^14 = 2^30 < 2^31
-constexpr int G2_STAGES = 4;
-constexpr int G2_OPERAND = 2 * 128 * 64;        // bytes of A (or B) per stage
-constexpr int G2_STAGE = 2 * G2_OPERAND;
+// Tile shapes: WM wave rows of 64 (WG tile 64*WM x 128, 2*WM waves) and a
+// ring of STAGES LDS stages; each stage holds 2 slices x 64*WM rows of A and
+// 2 slices x 128 rows of B, 64 bytes of K each.
+template <int WM> struct G2Shape {
+    static constexpr int M = 64 * WM, THREADS = 128 * WM, WAVES = 2 * WM;
+    static constexpr int A_BYTES = 2 * M * 64, STAGE = A_BYTES + 2 * 128 * 64;
+    static constexpr int A_DMA = 2 * M / 16, DMA = A_DMA + 16, DMA_PER_WAVE = DMA / WAVES;
+};
 
 typedef __attribute__((address_space(3))) void lds_void;
 
@@ -484,22 +491,34 @@ __device__ __forceinline__ void wait_vmcnt()
 {
     __builtin_amdgcn_s_waitcnt((N & 15) | (((N >> 4) & 3) << 14) | 0x0F70);
 }
+// wait until at most n * PER of this wave's DMAs are in flight (n < 4)
+template <int PER>
+__device__ __forceinline__ void wait_stages(int n)
+{
+    if (n >= 3) wait_vmcnt<3 * PER>();
+    else if (n == 2) wait_vmcnt<2 * PER>();
+    else if (n == 1) wait_vmcnt<PER>();
+    else wait_vmcnt<0>();
+}
 
-template <bool CHUNKED>
+template <bool CHUNKED, int WM, int STAGES>
 __device__ __forceinline__ void gemm_tile2(int64_t v,
     const PrimeConst *__restrict__ pc, int P, const int8_t *__restrict__ RA, const int8_t *__restrict__ RB,
     int64_t planeA, int64_t planeB, int64_t kpad, int64_t rowsC, int64_t colsC, int ntm, int ntn,
     uint32_t *__restrict__ Y, int64_t ystride)
 {
-    __shared__ __attribute__((aligned(1024))) int8_t LS[G2_STAGES * G2_STAGE];
-    uint32_t *LW = (uint32_t *)LS;   // after the K loop: the 128 x 128 output words
+    using Sh = G2Shape<WM>;
+    static_assert(Sh::DMA % Sh::WAVES == 0, "DMA instructions must split evenly over the waves");
+    static_assert(STAGES * Sh::STAGE >= Sh::M * 128 * 4, "the output words reuse the ring");
+    __shared__ __attribute__((aligned(1024))) int8_t LS[STAGES * Sh::STAGE];
+    uint32_t *LW = (uint32_t *)LS;   // after the K loop: the M x 128 output words
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wi = wave >> 1, wj = wave & 1, l32 = lane & 31, h = lane >> 5;
 
     const int64_t tiles = (int64_t)ntm * ntn;
     const int pair = (int)(v / tiles);
     const int64_t t = v % tiles;
-    const int64_t i0 = (t / ntn) * 128, j0 = (t % ntn) * 128;
+    const int64_t i0 = (t / ntn) * Sh::M, j0 = (t % ntn) * 128;
     uint32_t *Yw = Y + (int64_t)pair * ystride;
     const int nsteps = (int)(kpad / 64);
     const uint32_t ls0 = __builtin_amdgcn_readfirstlane(lds_offset(LS));
@@ -513,22 +532,26 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
         const uint32_t mu32 = (uint32_t)(c.mu >> 32), off32 = ((1u << 31) / c.p + 1) * c.p;
         const int8_t *gA = RA + (int64_t)q * 2 * planeA;
         const int8_t *gB = RB + (int64_t)q * 2 * planeB;
-        // this wave's 8 DMA sources per stage: u -> operand u>>2 ... see ISSUE
-        const int8_t *src[8];
+        // this wave's DMA sources per stage; DMA instruction idx fills 16 rows
+        // (1 KB): idx < A_DMA: A slice idx / (M/16), rows 16 (idx % (M/16));
+        // else B slice (idx - A_DMA) / 8, rows 16 ((idx - A_DMA) % 8)
+        const int8_t *src[Sh::DMA_PER_WAVE];
+        uint32_t dst[Sh::DMA_PER_WAVE];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const int idx = wave * 8 + u, op = idx >> 4, s = (idx >> 3) & 1, rb16 = idx & 7;
+        for (int u = 0; u < Sh::DMA_PER_WAVE; u++) {
+            const int idx = wave * Sh::DMA_PER_WAVE + u;
+            const bool isA = idx < Sh::A_DMA;
+            const int j = isA ? idx : idx - Sh::A_DMA, rbn = isA ? Sh::M / 16 : 8;
+            const int s = j / rbn, rb16 = j % rbn, rows = isA ? Sh::M : 128;
             const int row = rb16 * 16 + (lane >> 2), dchunk = (lane & 3) ^ g2_swz(row);
-            src[u] = op ? gB + s * planeB + (j0 + row) * kpad + 16 * dchunk
-                        : gA + s * planeA + (i0 + row) * kpad + 16 * dchunk;
+            src[u] = isA ? gA + s * planeA + (i0 + row) * kpad + 16 * dchunk
+                         : gB + s * planeB + (j0 + row) * kpad + 16 * dchunk;
+            dst[u] = (isA ? 0 : Sh::A_BYTES) + (s * rows + rb16 * 16) * 64;
         }
 #define ISSUE(k_)                                                                                     \
-        _Pragma("unroll") for (int u = 0; u < 8; u++) {                                               \
-            const int idx = wave * 8 + u, op = idx >> 4, s = (idx >> 3) & 1, rb16 = idx & 7;          \
+        _Pragma("unroll") for (int u = 0; u < Sh::DMA_PER_WAVE; u++)                                  \
             dma16(src[u] + (int64_t)(k_) * 64,                                                         \
-                  __builtin_amdgcn_readfirstlane(ls0 + ((k_) & 3) * G2_STAGE + op * G2_OPERAND +       \
-                                                 (s * 128 + rb16 * 16) * 64));                         \
-        }
+                  __builtin_amdgcn_readfirstlane(ls0 + ((k_) % STAGES) * Sh::STAGE + dst[u]));
 
         v16i acc[3][2][2];
 #pragma unroll
@@ -572,21 +595,18 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
         _Pragma("unroll") for (int ct = 0; ct < 2; ct++) acc[d][rt][ct] = (v16i){0};
 
         __syncthreads();   // every wave is done with the previous prime's stages
-        ISSUE(0);
-        if (1 < nsteps) { ISSUE(1); }
-        if (2 < nsteps) { ISSUE(2); }
+#pragma unroll
+        for (int k = 0; k < STAGES - 1; k++)
+            if (k < nsteps) { ISSUE(k); }
         int64_t since = 0;
         bool first = true;
         for (int k = 0; k < nsteps; k++) {
-            // stage k has landed once only the (up to two) later stages' DMAs are in flight
-            const int later = min(nsteps - 1, k + 2) - k;
-            if (later >= 2) wait_vmcnt<16>();
-            else if (later == 1) wait_vmcnt<8>();
-            else wait_vmcnt<0>();
+            // stage k has landed once only the later stages' DMAs (up to STAGES - 2) are in flight
+            wait_stages<Sh::DMA_PER_WAVE>(min(nsteps - 1, k + STAGES - 2) - k);
             __syncthreads();
-            // stage (k+3)&3 was last read in step k-1, which every wave has finished
-            if (k + 3 < nsteps) { ISSUE(k + 3); }
-            const int8_t *sa = LS + (k & 3) * G2_STAGE, *sb = sa + G2_OPERAND;
+            // stage (k + STAGES - 1) % STAGES was last read in step k-1, which every wave has finished
+            if (k + STAGES - 1 < nsteps) { ISSUE(k + STAGES - 1); }
+            const int8_t *sa = LS + (k % STAGES) * Sh::STAGE, *sb = sa + Sh::A_BYTES;
 #pragma unroll
             for (int kk = 0; kk < 2; kk++) {
                 const int ch = 2 * kk + h;
@@ -596,7 +616,7 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
 #pragma unroll
                     for (int x = 0; x < 2; x++) {
                         const int ra = wi * 64 + x * 32 + l32, rb = wj * 64 + x * 32 + l32;
-                        af[s][x] = *(const v4i *)(sa + (s * 128 + ra) * 64 + 16 * (ch ^ g2_swz(ra)));
+                        af[s][x] = *(const v4i *)(sa + (s * Sh::M + ra) * 64 + 16 * (ch ^ g2_swz(ra)));
                         bf[s][x] = *(const v4i *)(sb + (s * 128 + rb) * 64 + 16 * (ch ^ g2_swz(rb)));
                     }
 #pragma unroll
@@ -634,10 +654,10 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
                             (hold[rt][ct][r >> 1] >> (16 * (r & 1))) & 0xffff;
         }
         __syncthreads();
-        // two rows of 128 words per pass, 512 contiguous bytes per row
+        // THREADS/128 rows of 128 words per pass, 512 contiguous bytes per row
         const int64_t j = j0 + (tid & 127);
         for (int pass = 0; pass < 64; pass++) {
-            const int row = 2 * pass + (tid >> 7);
+            const int row = (Sh::THREADS / 128) * pass + (tid >> 7);
             const int64_t i = i0 + row;
             if (i < rowsC && j < colsC) Yw[i * colsC + j] = LW[row * 128 + (tid & 127)];
         }
@@ -662,13 +682,21 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
 // ------------------------------------------------------------------------
 constexpr int RECON_ROWS = 128;   // result bytes per workgroup
 constexpr int RECON_ENT = 256;    // entries per workgroup (64 per wave)
+// RESIDENT: the workgroup's whole row block of Mc (128 x Kr bytes) sits in
+// dynamic LDS, loaded once, and the K loop runs without barriers; used while
+// 128 (Kr + 16) bytes fit beside a second workgroup.
+constexpr int RECON_RES_MAX_LDS = 144 * 1024;
+inline int64_t recon_res_lds(int64_t Kr) { return (int64_t)RECON_ROWS * (Kr + 16); }
 
-template <int S>
+template <int S, bool RESIDENT>
 __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *__restrict__ pc, int P,
     const int8_t *__restrict__ Mc, int Kr, int nrb, int nw, const uint32_t *__restrict__ Y, int64_t ystride,
     int64_t nent, int64_t *__restrict__ Wv, uint64_t *__restrict__ G)
 {
-    __shared__ __attribute__((aligned(16))) int8_t LA[RECON_ROWS * LDS_PITCH];
+    __shared__ __attribute__((aligned(16))) int8_t LAs[RESIDENT ? 16 : RECON_ROWS * LDS_PITCH];
+    extern __shared__ __attribute__((aligned(16))) int8_t LDYN[];
+    int8_t *const LA = RESIDENT ? LDYN : LAs;
+    const int apitch = RESIDENT ? Kr + 16 : LDS_PITCH;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, h = lane >> 5, c32 = lane & 31;
     const int rb = (int)(blk % nrb);
     const int64_t e0 = (blk / nrb) * RECON_ENT;
@@ -704,13 +732,15 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
     }
 #define RSTEP(ycur, ynext2)                                                                          \
     {                                                                                                \
-        __syncthreads();                                                                             \
-        _Pragma("unroll") for (int u = 0; u < 2; u++) {                                              \
-            const int ch = tid + 256 * u;                                                            \
-            *(v4i *)(LA + (ch >> 2) * LDS_PITCH + 16 * (ch & 3)) = pa[u];                            \
+        if (!RESIDENT) {                                                                             \
+            __syncthreads();                                                                         \
+            _Pragma("unroll") for (int u = 0; u < 2; u++) {                                          \
+                const int ch = tid + 256 * u;                                                        \
+                *(v4i *)(LA + (ch >> 2) * LDS_PITCH + 16 * (ch & 3)) = pa[u];                        \
+            }                                                                                        \
+            __syncthreads();                                                                         \
+            if (k0 + 64 < Kr) { ALOAD(pa, k0 + 64); }                                                \
         }                                                                                            \
-        __syncthreads();                                                                             \
-        if (k0 + 64 < Kr) { ALOAD(pa, k0 + 64); }                                                    \
         if (k0 + 128 < Kr) { YLOAD(ynext2, k0 + 128); }                                              \
         if ((k0 / 64) % nrb == rb) {   /* this row block's share of the gamma sums */               \
             _Pragma("unroll") for (int kk = 0; kk < 2; kk++)                                         \
@@ -732,13 +762,22 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
         }                                                                                            \
         _Pragma("unroll") for (int kk = 0; kk < 2; kk++)                                             \
         _Pragma("unroll") for (int rt = 0; rt < 4; rt++) {                                           \
-            const v4i af = *(const v4i *)(LA + (rt * 32 + c32) * LDS_PITCH + kk * 32 + 16 * h);      \
+            const v4i af = *(const v4i *)(LA + (rt * 32 + c32) * apitch + (RESIDENT ? k0 : 0) + kk * 32 + 16 * h); \
             _Pragma("unroll") for (int ct = 0; ct < 2; ct++) acc[rt][ct] = mfma32(af, ycur[ct][kk], acc[rt][ct]); \
         }                                                                                            \
         k0 += 64;                                                                                    \
         if (k0 >= Kr) break;                                                                         \
     }
-    ALOAD(pa, 0);
+    if (RESIDENT) {
+        const int cpr = Kr / 16;
+        for (int c = tid; c < RECON_ROWS * cpr; c += 256) {
+            const int row = c / cpr, part = c - row * cpr;
+            *(v4i *)(LA + row * apitch + 16 * part) = *(const v4i *)(Mc + (int64_t)(rb * RECON_ROWS + row) * Kr + 16 * part);
+        }
+        __syncthreads();
+    } else {
+        ALOAD(pa, 0);
+    }
     YLOAD(y0, 0);
     if (64 < Kr) { YLOAD(y1, 64); }
     for (int k0 = 0;;) {
