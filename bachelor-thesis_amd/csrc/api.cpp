@@ -46,14 +46,6 @@ int ensure_init()
         MGX_HIP(hipMalloc(&w, 256));
         MGX_HIP(hipMemset(w, 0, 256));
         g_errword[dev] = w;
-        // stream-ordered scratch (residues, Y, column sums: gigabytes per
-        // product) stays mapped in the device's pool between calls instead of
-        // going back to the driver at every synchronization
-        hipMemPool_t pool;
-        if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
-            uint64_t keep = ~0ull;
-            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
-        }
     }
     g_init = true;
     return MGX_OK;
@@ -86,17 +78,109 @@ int check_device_error(hipStream_t s)
     return MGX_OK;
 }
 
+// Scratch blocks are cached per device for the life of the process (or until
+// mgx_scratch_release): a product's residues, Y planes and column sums take
+// over a gigabyte, and mapping that afresh costs the better part of a
+// millisecond per call.  A block freed on a stream carries an event recorded
+// there; whichever stream takes it next waits on that event, so reuse stays
+// stream-ordered without a host synchronization.  A request takes the smallest
+// idle block that holds it and is at most about twice its size; otherwise a
+// new block is mapped, and idle blocks that the new one supersedes (large ones
+// smaller than it) are unmapped.  Work being captured into a graph gets
+// graph-owned memory from the stream-ordered allocator instead.
+namespace {
+struct ScratchBlock {
+    void *p;
+    size_t size;
+    hipEvent_t ev;   // recorded where the block was last released
+    bool busy;
+};
+std::mutex g_scratch_mu;
+std::vector<std::vector<ScratchBlock>> g_scratch;   // per device
+constexpr size_t SCRATCH_SLACK = 4u << 20;
+constexpr size_t SCRATCH_MAX_BLOCKS = 64;
+
+void unmap_block(ScratchBlock &b)
+{
+    (void)hipEventSynchronize(b.ev);
+    (void)hipEventDestroy(b.ev);
+    (void)hipFree(b.p);
+}
+}  // namespace
+
 int scratch_alloc(void **p, size_t bytes, hipStream_t s)
 {
-    if (bytes == 0) bytes = 256;
-    hipError_t e = hipMallocAsync(p, bytes, s);
-    if (e != hipSuccess) return fail(MGX_ERR_OOM, "hipMallocAsync(%zu): %s", bytes, hipGetErrorString(e));
+    bytes = (size_t)round_up((int64_t)std::max<size_t>(bytes, 1), 256);
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    MGX_HIP(hipStreamIsCapturing(s, &cap));
+    if (cap != hipStreamCaptureStatusNone) {
+        hipError_t e = hipMallocAsync(p, bytes, s);
+        if (e != hipSuccess) return fail(MGX_ERR_OOM, "hipMallocAsync(%zu): %s", bytes, hipGetErrorString(e));
+        return MGX_OK;
+    }
+    int dev = 0;
+    MGX_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1);
+    auto &blocks = g_scratch[dev];
+    ScratchBlock *best = nullptr;
+    for (auto &b : blocks)
+        if (!b.busy && b.size >= bytes && b.size <= 2 * bytes + SCRATCH_SLACK && (!best || b.size < best->size))
+            best = &b;
+    if (best) {
+        MGX_HIP(hipStreamWaitEvent(s, best->ev, 0));
+        best->busy = true;
+        *p = best->p;
+        return MGX_OK;
+    }
+    for (size_t i = blocks.size(); i-- > 0;) {
+        ScratchBlock &b = blocks[i];
+        bool superseded = b.size > SCRATCH_SLACK && b.size < bytes;
+        if (!b.busy && (superseded || blocks.size() >= SCRATCH_MAX_BLOCKS)) {
+            unmap_block(b);
+            blocks.erase(blocks.begin() + (ptrdiff_t)i);
+        }
+    }
+    ScratchBlock nb{nullptr, bytes, nullptr, true};
+    hipError_t e = hipMalloc(&nb.p, bytes);
+    if (e != hipSuccess) return fail(MGX_ERR_OOM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    e = hipEventCreateWithFlags(&nb.ev, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        (void)hipFree(nb.p);
+        return fail(MGX_ERR_HIP, "hipEventCreateWithFlags: %s", hipGetErrorString(e));
+    }
+    blocks.push_back(nb);
+    *p = nb.p;
     return MGX_OK;
 }
 
 int scratch_free(void *p, hipStream_t s)
 {
-    if (p) MGX_HIP(hipFreeAsync(p, s));
+    if (!p) return MGX_OK;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        for (auto &blocks : g_scratch)
+            for (auto &b : blocks)
+                if (b.p == p && b.busy) {
+                    MGX_HIP(hipEventRecord(b.ev, s));
+                    b.busy = false;
+                    return MGX_OK;
+                }
+    }
+    MGX_HIP(hipFreeAsync(p, s));   // graph-owned memory from a captured call
+    return MGX_OK;
+}
+
+int scratch_release_device(int dev)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch.size() <= dev) return MGX_OK;
+    auto &blocks = g_scratch[dev];
+    for (size_t i = blocks.size(); i-- > 0;)
+        if (!blocks[i].busy) {
+            unmap_block(blocks[i]);
+            blocks.erase(blocks.begin() + (ptrdiff_t)i);
+        }
     return MGX_OK;
 }
 
@@ -140,6 +224,12 @@ int mgx_free(void *p)
 {
     if (p) MGX_HIP(hipFree(p));
     return MGX_OK;
+}
+int mgx_scratch_release(void)
+{
+    int dev = 0;
+    MGX_HIP(hipGetDevice(&dev));
+    return scratch_release_device(dev);
 }
 int mgx_memcpy_htod(void *dst, const void *src, size_t bytes, void *stream)
 {

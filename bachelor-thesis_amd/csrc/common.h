@@ -34,9 +34,11 @@ enum : unsigned { DEV_ERR_CAPACITY = 1u, DEV_ERR_INPUT = 2u, DEV_ERR_WIDTH = 4u 
 unsigned *device_error_word();          // per-device, zero-initialised at init
 int check_device_error(hipStream_t s);  // synchronizes s, reads and clears the word
 
-// Stream-ordered scratch from the device's default memory pool.
+// Stream-ordered scratch: blocks cached per device and reused behind an
+// event recorded at release (graph-owned pool memory while a stream captures).
 int scratch_alloc(void **p, size_t bytes, hipStream_t s);
 int scratch_free(void *p, hipStream_t s);
+int scratch_release_device(int dev);   // unmaps the device's idle cached blocks
 
 int ensure_init();
 

@@ -38,23 +38,36 @@ namespace mgx {
 
 // ------------------------------------------------------------------------
 // bit-length scan: out[0] = max over entries of bit length of |x|; one
-// atomic per workgroup
+// atomic per workgroup.  Only the size word and the top limb of an entry are
+// read; each thread has the loads of four entries in flight at once (the scan
+// is latency-bound: two dependent 8-byte loads per 536-byte entry).
 // ------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void scan_bits_kernel(const uint64_t *X, int64_t rs, int64_t W, int64_t rows,
                                                         int64_t cols, int *out)
 {
     __shared__ int s_best[4];
-    int64_t n = rows * cols;
+    const int64_t n = rows * cols, stride = (int64_t)gridDim.x * blockDim.x;
     int best = 0;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-        int64_t r = e / cols, c = e % cols;
-        const uint64_t *p = X + (r * rs + c) * W;
-        int64_t s = (int64_t)p[W - 1];
-        int64_t a = min(s < 0 ? -s : s, W - 1);
-        while (a > 0 && p[a - 1] == 0) a--;
-        if (a > 0) {
-            int b = (int)((a - 1) * 64 + 64 - __clzll(p[a - 1]));
-            best = max(best, b);
+    for (int64_t e0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e0 < n; e0 += 4 * stride) {
+        const uint64_t *p[4];
+        int64_t a[4];
+        uint64_t top[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            int64_t e = e0 + j * stride;
+            p[j] = X + ((e / cols) * rs + e % cols) * W;
+            int64_t s = e < n ? (int64_t)p[j][W - 1] : 0;
+            a[j] = min(s < 0 ? -s : s, W - 1);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) top[j] = a[j] > 0 ? p[j][a[j] - 1] : 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            while (a[j] > 0 && top[j] == 0) {   // zero top limbs: not normalized
+                a[j]--;
+                top[j] = a[j] > 0 ? p[j][a[j] - 1] : 0;
+            }
+            if (a[j] > 0) best = max(best, (int)((a[j] - 1) * 64 + 64 - __clzll(top[j])));
         }
     }
     for (int o = 32; o > 0; o >>= 1) best = max(best, __shfl_xor(best, o));
@@ -521,7 +534,7 @@ int scan_bits_pair(int nmat, const int64_t *rows, const int64_t *cols, const mgx
     MGX_HIP(hipMemsetAsync(d, 0, 2 * sizeof(int), s));
     for (int m = 0; m < nmat; m++) {
         int64_t n = rows[m] * cols[m];
-        int grid = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(n, 256), 1), 1024);
+        int grid = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(n, 1024), 1), 2048);
         if (n > 0)
             hipLaunchKernelGGL(scan_bits_kernel, dim3(grid), dim3(256), 0, s, (const uint64_t *)X[m], Xrs[m], cap + 1,
                                rows[m], cols[m], d + m);

@@ -47,6 +47,7 @@ def _declare(L):
         "mgx_check_error": [c_vp],
         "mgx_malloc": [ctypes.POINTER(c_vp), ctypes.c_size_t],
         "mgx_free": [c_vp],
+        "mgx_scratch_release": [],
         "mgx_memcpy_htod": [c_vp, c_vp, ctypes.c_size_t, c_vp],
         "mgx_memcpy_dtoh": [c_vp, c_vp, ctypes.c_size_t, c_vp],
         "mgx_memset": [c_vp, ctypes.c_int, ctypes.c_size_t, c_vp],

@@ -70,6 +70,11 @@ int mgx_check_error(void *stream);
  * other HIP binding). */
 int mgx_malloc(void **p, size_t bytes);
 int mgx_free(void *p);
+/* The library keeps the scratch blocks of its products (residues, Y planes,
+ * recursion temporaries) mapped between calls; this unmaps the current
+ * device's idle ones.  Housekeeping with no counterpart in the reference,
+ * which allocates per call (mat-gpu/mat-gpu.cu:98). */
+int mgx_scratch_release(void);
 int mgx_memcpy_htod(void *dst, const void *src, size_t bytes, void *stream);
 int mgx_memcpy_dtoh(void *dst, const void *src, size_t bytes, void *stream);
 int mgx_memset(void *dst, int value, size_t bytes, void *stream);
