@@ -340,7 +340,12 @@ int mgx_crt_plan_create(mgx_crt_plan **out, int64_t K, int64_t bitsA, int64_t bi
         for (int d = 0; d < 7; d++) {
             uint64_t w = powmod64(2, 8 * d, p);
             c.w[d] = w > p / 2 ? (int32_t)w - (int32_t)p : (int32_t)w;
+            if (d < 3) {
+                uint64_t wv = w * c.inv % p;
+                c.wi[d] = wv > p / 2 ? (int32_t)wv - (int32_t)p : (int32_t)wv;
+            }
         }
+        c.pinv = 1.0 / (double)p;
         uint64_t ks = 0, pw = 1;
         for (int64_t b = 0; b < pl->kb; b++) {
             // reduce-GEMM constants: digits of centred 2^(8b) mod p

@@ -25,6 +25,8 @@ struct PrimeConst {
     uint64_t fx;     // floor(2^63 / p)
     int32_t w[7];    // 2^(8d) mod p, centred
     uint32_t kofs;   // 128 * sum_{b<kb} 2^(8b) mod p
+    int32_t wi[3];   // 2^(8d) (M/p)^-1 mod p, centred
+    double pinv;     // 1/p, correctly rounded
 };
 
 __device__ __forceinline__ uint32_t mod_u64(uint64_t u, uint32_t p, uint64_t mu)
@@ -69,11 +71,12 @@ __device__ __forceinline__ int32_t unslice4(uint32_t w)
            ((int32_t)(int8_t)((w >> 16) & 255) << 16) + ((int32_t)(int8_t)(w >> 24) << 24);
 }
 __device__ __forceinline__ int32_t centre(uint32_t r, uint32_t p) { return r > (p >> 1) ? (int32_t)r - (int32_t)p : (int32_t)r; }
-// two balanced digits of |r| <= 32639 in the low 16 bits
+// two balanced digits of -32896 <= r <= 32639 in the low 16 bits: with
+// u = r + 128, the low digit's byte is (u & 255) ^ 128 and the high digit is
+// u >> 8 (arithmetic), i.e. bits 8..15 of u
 __device__ __forceinline__ uint32_t slice2(int32_t r)
 {
-    const int32_t t = ((r + 128) & 255) - 128;
-    return (uint32_t)(t & 255) | ((uint32_t)((r - t) >> 8) & 255) << 8;
+    return ((uint32_t)(r + 128) ^ 0x80u) & 0xffffu;
 }
 __device__ __forceinline__ int32_t unslice2(uint32_t h)
 {
@@ -477,13 +480,23 @@ typedef __attribute__((address_space(3))) void lds_void;
 __device__ __forceinline__ uint32_t lds_offset(const void *p) { return (uint32_t)(uintptr_t)(const lds_void *)p; }
 __device__ __forceinline__ int g2_swz(int r) { return ((r >> 1) ^ (r >> 3)) & 3; }
 
-// 16 bytes from g (per lane) to LDS at base + 16 * lane; base is wave-uniform.
-// Issued as inline assembly so that the compiler does not wait on it: the
-// caller orders it with wait_vmcnt and a barrier.
-__device__ __forceinline__ void dma16(const void *g, uint32_t lds_base)
+// 16 bytes from sbase + voff (sbase wave-uniform, voff per lane) to LDS at
+// lds_base + 16 * lane (lds_base wave-uniform).  Issued as inline assembly so
+// that the compiler does not wait on it: the caller orders it with
+// wait_vmcnt and a barrier.
+__device__ __forceinline__ void dma16(const int8_t *sbase, uint32_t voff, uint32_t lds_base)
 {
-    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(lds_base), "v"(g)
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, %2" ::"s"(lds_base), "v"(voff),
+                 "s"(sbase)
                  : "memory", "m0");
+}
+// a pointer that is the same in every lane, as the compiler can see it
+__device__ __forceinline__ const int8_t *uniform_ptr(const int8_t *p)
+{
+    const uint64_t v = (uint64_t)(uintptr_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (const int8_t *)(uintptr_t)(((uint64_t)hi << 32) | lo);
 }
 // wait until at most N vector memory operations of this wave are in flight
 template <int N>
@@ -513,6 +526,7 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
     __shared__ __attribute__((aligned(1024))) int8_t LS[STAGES * Sh::STAGE];
     uint32_t *LW = (uint32_t *)LS;   // after the K loop: the M x 128 output words
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wave_u = __builtin_amdgcn_readfirstlane(wave);
     const int wi = wave >> 1, wj = wave & 1, l32 = lane & 31, h = lane >> 5;
 
     const int64_t tiles = (int64_t)ntm * ntn;
@@ -529,28 +543,33 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
         const int q = 2 * pair + half;
         if (q >= P) break;
         const PrimeConst c = pc[q];
-        const uint32_t mu32 = (uint32_t)(c.mu >> 32), off32 = ((1u << 31) / c.p + 1) * c.p;
+        const double pd = (double)c.p, pinv = c.pinv;
+        const double mw1 = (double)c.w[1], mw2 = (double)c.w[2];
+        const double fw0 = (double)c.wi[0], fw1 = (double)c.wi[1], fw2 = (double)c.wi[2];
         const int8_t *gA = RA + (int64_t)q * 2 * planeA;
         const int8_t *gB = RB + (int64_t)q * 2 * planeB;
         // this wave's DMA sources per stage; DMA instruction idx fills 16 rows
         // (1 KB): idx < A_DMA: A slice idx / (M/16), rows 16 (idx % (M/16));
-        // else B slice (idx - A_DMA) / 8, rows 16 ((idx - A_DMA) % 8)
-        const int8_t *src[Sh::DMA_PER_WAVE];
-        uint32_t dst[Sh::DMA_PER_WAVE];
+        // else B slice (idx - A_DMA) / 8, rows 16 ((idx - A_DMA) % 8).  The
+        // 16-row block's start is a wave-uniform base, the lane's row and
+        // chunk a 32-bit offset.
+        const int8_t *sbase[Sh::DMA_PER_WAVE];
+        uint32_t voff[Sh::DMA_PER_WAVE], dst[Sh::DMA_PER_WAVE];
 #pragma unroll
         for (int u = 0; u < Sh::DMA_PER_WAVE; u++) {
-            const int idx = wave * Sh::DMA_PER_WAVE + u;
+            const int idx = wave_u * Sh::DMA_PER_WAVE + u;
             const bool isA = idx < Sh::A_DMA;
             const int j = isA ? idx : idx - Sh::A_DMA, rbn = isA ? Sh::M / 16 : 8;
             const int s = j / rbn, rb16 = j % rbn, rows = isA ? Sh::M : 128;
             const int row = rb16 * 16 + (lane >> 2), dchunk = (lane & 3) ^ g2_swz(row);
-            src[u] = isA ? gA + s * planeA + (i0 + row) * kpad + 16 * dchunk
-                         : gB + s * planeB + (j0 + row) * kpad + 16 * dchunk;
+            sbase[u] = uniform_ptr(isA ? gA + s * planeA + (i0 + rb16 * 16) * kpad
+                                       : gB + s * planeB + (j0 + rb16 * 16) * kpad);
+            voff[u] = (uint32_t)((lane >> 2) * kpad + 16 * dchunk);
             dst[u] = (isA ? 0 : Sh::A_BYTES) + (s * rows + rb16 * 16) * 64;
         }
 #define ISSUE(k_)                                                                                     \
         _Pragma("unroll") for (int u = 0; u < Sh::DMA_PER_WAVE; u++)                                  \
-            dma16(src[u] + (int64_t)(k_) * 64,                                                         \
+            dma16(sbase[u] + (int64_t)(k_) * 64, voff[u],                                              \
                   __builtin_amdgcn_readfirstlane(ls0 + ((k_) % STAGES) * Sh::STAGE + dst[u]));
 
         v16i acc[3][2][2];
@@ -561,25 +580,30 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
 #pragma unroll
                 for (int ct = 0; ct < 2; ct++) acc[d][rt][ct] = (v16i){0};
 
-        // x = acc0 + 256 (acc1 + 256 acc2) mod p plus the running residue of
-        // earlier chunks, in 32-bit Barrett steps (|acc_d| <= 2^30 within a
-        // chunk); the last fold turns it into y = x (M/p)^-1 mod p, centred,
-        // as two balanced digits
-#define FOLD2(first_, last_)                                                                         \
+        // y = (acc0 + prev) w0 + acc1 w1 + acc2 w2 with w_d = 2^(8d) mod p,
+        // times (M/p)^-1 at the last fold, centred (|w_d| < 2^15); prev is the
+        // centred residue of earlier chunks.  |acc_d| <= 2^30 within a chunk,
+        // so every partial sum is an integer below 2^47 and exact in double.
+        // y/p is at least 1/(2p) > 2^-17 away from a half-integer (p odd) and
+        // |y/p| <= 2^32 makes y * pinv off by less than 2^-19, so rint gives
+        // the nearest quotient and y - rint(y pinv) p is the centred residue,
+        // exactly.
+#define FOLD2(first_, last_, h_)                                                                     \
         _Pragma("unroll") for (int rt = 0; rt < 2; rt++)                                             \
         _Pragma("unroll") for (int ct = 0; ct < 2; ct++)                                             \
         _Pragma("unroll") for (int r = 0; r < 16; r++) {                                             \
             const int64_t i = i0 + wi * 64 + rt * 32 + 8 * (r >> 2) + 4 * h + (r & 3);               \
             const int64_t j = j0 + wj * 64 + ct * 32 + l32;                                          \
-            uint16_t *yh = (uint16_t *)(Yw + i * colsC + j) + half;                                  \
+            uint16_t *yh = (uint16_t *)(Yw + i * colsC + j) + (h_);                                  \
             const bool in = i < rowsC && j < colsC;                                                  \
-            const int32_t prev = (first_ || !in) ? 0 : (int32_t)*yh;                                 \
-            const uint32_t r2 = smod32(acc[2][rt][ct][r], c.p, mu32, off32);                        \
-            const uint32_t z = smod32(acc[1][rt][ct][r] + 256 * (int32_t)r2, c.p, mu32, off32);     \
-            uint32_t res = smod32(acc[0][rt][ct][r] + 256 * (int32_t)z + prev, c.p, mu32, off32);   \
-            if (last_) res = slice2(centre(bred32(res * c.inv, c.p, mu32), c.p));                    \
+            const int32_t prev = (first_ || !in) ? 0 : (int32_t)(int16_t)*yh;                        \
+            const double y = fma((double)acc[2][rt][ct][r], (last_) ? fw2 : mw2,                     \
+                                 fma((double)acc[1][rt][ct][r], (last_) ? fw1 : mw1,                 \
+                                     (double)(acc[0][rt][ct][r] + prev) * ((last_) ? fw0 : 1.0)));   \
+            const int32_t cr = (int32_t)fma(-rint(y * pinv), pd, y);                                 \
+            const uint32_t res = (last_) ? slice2(cr) : ((uint32_t)cr & 0xffffu);                    \
             if (!CHUNKED && last_) {                                                                 \
-                if (half == 0) {                                                                     \
+                if ((h_) == 0) {                                                                     \
                     if (r & 1) hold[rt][ct][r >> 1] |= res << 16;                                    \
                     else hold[rt][ct][r >> 1] = res;                                                 \
                 } else {                                                                             \
@@ -631,14 +655,20 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
             }
             since += 64;
             if (CHUNKED && since == KCHUNK2 && k + 1 < nsteps) {
-                FOLD2(first, false);
+                FOLD2(first, false, half);
                 first = false;
                 since = 0;
             }
         }
         __syncthreads();   // the output words reuse the stages
-        if (first) { FOLD2(true, true); }
-        else { FOLD2(false, true); }
+        if (CHUNKED) {
+            if (first) { FOLD2(true, true, half); }
+            else { FOLD2(false, true, half); }
+        } else if (half == 0) {   // one K chunk: this is the only fold
+            FOLD2(true, true, 0);
+        } else {
+            FOLD2(true, true, 1);
+        }
     }
 #undef ISSUE
 #undef FOLD2

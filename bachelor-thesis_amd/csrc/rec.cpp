@@ -2,9 +2,9 @@
 // Winograd, Winograd-addmul), squarify for rectangular shapes, and the
 // host-memory entry with mat-gpu's method codes.
 //
-// Everything is stream-ordered: temporaries come from hipMallocAsync and are
-// released with hipFreeAsync, no host synchronization happens inside the
-// recursion (the reference allocates and synchronizes after every launch,
+// Everything is stream-ordered: temporaries come from the library's scratch
+// cache and go back to it behind an event, no host synchronization happens
+// inside the recursion (the reference allocates and synchronizes after every launch,
 // mat-gpu/mat-gpu.cu:20,389).  The public matrix-level calls synchronize once
 // at the end and report the device error word as a status.
 #include "common.h"

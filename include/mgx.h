@@ -138,8 +138,8 @@ int mgx_naive_smallrect(int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Cr
 
 /* Square N x N recursion over 2x2 blocks down to the leaf size (the
  * reference's BASE_SIZE = 256, mat-gpu/mat-params.h:5; here a tunable,
- * mgx_set_param(MGX_PARAM_LEAF, ...)).  Temporaries come from a
- * stream-ordered pool; no host synchronization inside. */
+ * mgx_set_param(MGX_PARAM_LEAF, ...)).  Temporaries come from the
+ * library's stream-ordered scratch cache; no host synchronization inside. */
 typedef enum {
     MGX_REC_NAIVE = 0,          /* naive_recursive_gpu (mat-gpu.cu:79-164) */
     MGX_REC_STRASSEN = 1,       /* strassen_gpu (mat-gpu.cu:166-265) */

@@ -194,3 +194,43 @@ def test_plain_equals_crt_config3_sized_block(gpu):
     L = np.abs(sz)
     mask = np.arange(cap)[None, None, :] < L[..., None]
     assert (np.where(mask, P[..., :cap], 0) == np.where(mask, Q[..., :cap], 0)).all()
+
+
+def test_scratch_reuse_across_streams(gpu):
+    # Recursion temporaries come from the library's block cache: blocks released
+    # on one stream are taken by work queued on another without a host
+    # synchronization in between; every product must still be exact, also
+    # after the idle blocks are unmapped.
+    mgx = gpu
+    old = mgx.lib().mgx_get_param(0)
+    mgx.call("mgx_set_param", 0, 8)
+    streams = [mgx.c_vp(), mgx.c_vp()]
+    for s in streams:
+        mgx.call("mgx_stream_create", mgx.ctypes.byref(s))
+    try:
+        bits, cap = 300, 13
+        jobs = []
+        for i, (n, si) in enumerate([(64, 0), (32, 1), (64, 1), (32, 0), (64, 0)]):
+            A, B = rand_ints(n, n, bits, 900 + i), rand_ints(n, n, bits, 950 + i)
+            dA, dB = dev(mgx, to_entries(A, cap)), dev(mgx, to_entries(B, cap))
+            dC = mgx.DeviceBuffer(n * n * (cap + 1) * 8)
+            dC.zero()
+            jobs.append((n, A, B, dA, dB, dC, streams[si].value))
+        mgx.call("mgx_device_sync")
+        for n, A, B, dA, dB, dC, s in jobs:
+            mgx.call("mgx_recursive", 2, 0, n, dC.ptr, n, dA.ptr, n, dB.ptr, n, cap, s)
+        for s in streams:
+            mgx.call("mgx_stream_sync", s.value)
+        for n, A, B, dA, dB, dC, s in jobs:
+            assert to_ints(dC.to_array((n, n, cap + 1))) == matmul(A, B)
+        mgx.call("mgx_scratch_release")
+        n, A, B, dA, dB, dC, s = jobs[0]
+        dC.zero()
+        mgx.call("mgx_device_sync")
+        mgx.call("mgx_recursive", 1, 0, n, dC.ptr, n, dA.ptr, n, dB.ptr, n, cap, s)
+        mgx.call("mgx_stream_sync", s)
+        assert to_ints(dC.to_array((n, n, cap + 1))) == matmul(A, B)
+    finally:
+        for s in streams:
+            mgx.call("mgx_stream_destroy", s.value)
+        mgx.call("mgx_set_param", 0, old)
