@@ -346,6 +346,7 @@ int mgx_crt_plan_create(mgx_crt_plan **out, int64_t K, int64_t bitsA, int64_t bi
             }
         }
         c.pinv = 1.0 / (double)p;
+        c.off32 = p < 65536 ? (uint32_t)(((1ull << 31) / p + 1) * p) : 0;
         uint64_t ks = 0, pw = 1;
         for (int64_t b = 0; b < pl->kb; b++) {
             // reduce-GEMM constants: digits of centred 2^(8b) mod p

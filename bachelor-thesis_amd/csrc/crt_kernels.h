@@ -26,6 +26,7 @@ struct PrimeConst {
     int32_t w[7];    // 2^(8d) mod p, centred
     uint32_t kofs;   // 128 * sum_{b<kb} 2^(8b) mod p
     int32_t wi[3];   // 2^(8d) (M/p)^-1 mod p, centred
+    uint32_t off32;  // a multiple of p in (2^31, 2^31 + p] (p < 2^16)
     double pinv;     // 1/p, correctly rounded
 };
 
@@ -215,95 +216,123 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
     __shared__ __attribute__((aligned(16))) int8_t LO[64 * 128];
     __shared__ int64_t s_base[128];   // entry offset in words, -1 for padding
     __shared__ int s_nl[128];         // limbs to read, ~limbs when negative
-    __shared__ uint32_t s_pc[32][4];  // S = 2: p, kofs, floor(2^32/p), a multiple of p in [2^31, 2^31 + p)
+    __shared__ __attribute__((aligned(16))) uint32_t s_pc[32][4];   // S = 2: 1/p (two words), p, kofs
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t e0 = blk * 128;
     const int cpe = kb / 16;   // 16-byte chunks per entry
 
     if (tid < 128) {
         const int64_t e = e0 + tid, r = e / kpad, k = e % kpad;
-        int64_t base = -1;
-        int nl = 0;
-        bool neg = false;
-        if (r < rows && k < K) {
-            base = transpose ? (k * Xrs + r) * W : (r * Xrs + k) * W;
-            const int64_t s = (int64_t)X[base + W - 1];
-            const int64_t a = min(s < 0 ? -s : s, W - 1);
-            for (int64_t x = kb / 8; x < a; x++)   // limbs beyond the plan's bound must be zero
-                if (X[base + x]) { atomicOr(err, DEV_ERR_INPUT); break; }
-            nl = (int)min<int64_t>(a, kb / 8);
-            neg = s < 0;
-        }
-        s_base[tid] = base;
-        s_nl[tid] = neg ? ~nl : nl;
+        s_base[tid] = (r < rows && k < K) ? (transpose ? (k * Xrs + r) * W : (r * Xrs + k) * W) : -1;
     }
     __syncthreads();
-    for (int c = tid; c < 128 * cpe; c += 256) {
-        const int el = c / cpe, part = c - el * cpe;
-        const int64_t base = s_base[el];
-        const int v = s_nl[el], nl = v < 0 ? ~v : v;
-        uint64_t v0 = 0, v1 = 0;
-        if (2 * part < nl) v0 = X[base + 2 * part];
-        if (2 * part + 1 < nl) v1 = X[base + 2 * part + 1];
-        uint64_t *d = (uint64_t *)(LE + el * RES_PITCH + 16 * part);
-        d[0] = v0 ^ 0x8080808080808080ull;   // byte - 128
-        d[1] = v1 ^ 0x8080808080808080ull;
+    // thread c of the 128 x cpe chunks (at most 8 per thread) reads its entry's
+    // size word and the chunk's two limbs with no condition, so that all the
+    // loads are in flight at once (padding reads entry 0, limb indices are
+    // clamped below the entry's cap); what lies at or above |size| is masked
+    // afterwards
+    {
+        const int lim = (int)min<int64_t>(kb / 8, W - 1);
+        uint64_t v[8][2];
+        int64_t sz[8], bs[8];
+#pragma unroll
+        for (int it = 0; it < 8; it++) {
+            const int c = tid + 256 * it, el = min(c / cpe, 127), part = c - (c / cpe) * cpe;
+            bs[it] = c < 128 * cpe ? s_base[el] : -1;
+            const int64_t b0 = bs[it] >= 0 ? bs[it] : 0;
+            sz[it] = (int64_t)X[b0 + W - 1];
+            v[it][0] = X[b0 + min(2 * part, lim - 1)];
+            v[it][1] = X[b0 + min(2 * part + 1, lim - 1)];
+        }
+#pragma unroll
+        for (int it = 0; it < 8; it++) {
+            const int c = tid + 256 * it, el = c / cpe, part = c - el * cpe;
+            if (c < 128 * cpe) {
+                const int64_t a = bs[it] >= 0 ? min(sz[it] < 0 ? -sz[it] : sz[it], W - 1) : 0;
+                const int nl = (int)min<int64_t>(a, lim);
+                uint64_t *d = (uint64_t *)(LE + el * RES_PITCH + 16 * part);
+                d[0] = (2 * part < nl ? v[it][0] : 0) ^ 0x8080808080808080ull;   // byte - 128
+                d[1] = (2 * part + 1 < nl ? v[it][1] : 0) ^ 0x8080808080808080ull;
+                if (part == 0) {
+                    s_nl[el] = (bs[it] >= 0 && sz[it] < 0) ? ~nl : nl;
+                    for (int64_t x = kb / 8; x < a; x++)   // limbs beyond the plan's bound must be zero
+                        if (X[bs[it] + x]) { atomicOr(err, DEV_ERR_INPUT); break; }
+                }
+            }
+        }
     }
     const int el = wave * 32 + (lane & 31);
-    // the constants of row block rb: 64 rows x kb bytes, at most four 16-byte chunks per thread
+    // the constants of row block rb: 64 rows x kb bytes, at most four 16-byte
+    // chunks per thread, one block ahead in registers
     v4i cpre[4];
+    int cst_off[4], lc_off[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int c = tid + 256 * u, row = c / cpe, part = c - row * cpe;
+        cst_off[u] = c < 64 * cpe ? row * kb + 16 * part : -1;
+        lc_off[u] = row * RES_PITCH + 16 * part;
+    }
 #define CLOAD(rb_)                                                                                    \
-    _Pragma("unroll") for (int u = 0; u < 4; u++) {                                                   \
-        const int c = tid + 256 * u, row = c / cpe, part = c - row * cpe;                             \
-        if (c < 64 * cpe) cpre[u] = *(const v4i *)(cst + (int64_t)((rb_) * 64 + row) * kb + 16 * part); \
+    _Pragma("unroll") for (int u = 0; u < 4; u++)                                                     \
+        if (cst_off[u] >= 0) cpre[u] = *(const v4i *)(cst + (int64_t)(rb_) * 64 * kb + cst_off[u]);
+    // S = 2: the row block's 32 primes' constants, one block ahead in registers
+    uint32_t pcv[4] = {0, 0, 0, 0};
+#define PCLOAD(rb_)                                                                                   \
+    if (S == 2 && tid < 32) {                                                                         \
+        const PrimeConst &c_ = pc[min((rb_) * 32 + tid, P - 1)];                                      \
+        const uint64_t pi_ = __double_as_longlong(c_.pinv);                                           \
+        pcv[0] = (uint32_t)pi_; pcv[1] = (uint32_t)(pi_ >> 32); pcv[2] = c_.p; pcv[3] = c_.kofs;        \
     }
     CLOAD(0);
+    PCLOAD(0);
     for (int rb = 0; rb < nrb; rb++) {
         __syncthreads();   // LE staged; LC, LO and s_pc free again
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int c = tid + 256 * u, row = c / cpe, part = c - row * cpe;
-            if (c < 64 * cpe) *(v4i *)(LC + row * RES_PITCH + 16 * part) = cpre[u];
-        }
-        if (S == 2 && tid < 32) {
-            const int q = min(rb * 32 + tid, P - 1);
-            const uint32_t p = pc[q].p;
-            s_pc[tid][0] = p;
-            s_pc[tid][1] = pc[q].kofs;
-            s_pc[tid][2] = (uint32_t)(pc[q].mu >> 32);
-            s_pc[tid][3] = ((1u << 31) / p + 1) * p;
-        }
+        for (int u = 0; u < 4; u++)
+            if (cst_off[u] >= 0) *(v4i *)(LC + lc_off[u]) = cpre[u];
+        if (S == 2 && tid < 32) *(uint4 *)s_pc[tid] = make_uint4(pcv[0], pcv[1], pcv[2], pcv[3]);
         __syncthreads();
-        if (rb + 1 < nrb) { CLOAD(rb + 1); }
-        v16i acc[2] = {(v16i){0}, (v16i){0}};
-        for (int k32 = 0; k32 < kb / 32; k32++) {
-            const int ko = k32 * 32 + 16 * (lane >> 5);
-            const v4i bf = *(const v4i *)(LE + el * RES_PITCH + ko);
+        if (rb + 1 < nrb) {
+            CLOAD(rb + 1);
+            PCLOAD(rb + 1);
+        }
+        v16i acc[2];
 #pragma unroll
-            for (int rt = 0; rt < 2; rt++) {
-                const v4i af = *(const v4i *)(LC + (rt * 32 + (lane & 31)) * RES_PITCH + ko);
-                acc[rt] = mfma32(af, bf, acc[rt]);
+        for (int k32 = 0; k32 < RES_KB / 32; k32++) {
+            if (k32 * 32 < kb) {
+                const int ko = k32 * 32 + 16 * (lane >> 5);
+                const v4i bf = *(const v4i *)(LE + el * RES_PITCH + ko);
+#pragma unroll
+                for (int rt = 0; rt < 2; rt++) {
+                    const v4i af = *(const v4i *)(LC + (rt * 32 + (lane & 31)) * RES_PITCH + ko);
+                    acc[rt] = mfma32(af, bf, k32 == 0 ? (v16i){0} : acc[rt]);
+                }
             }
         }
+        // the residue digits of rows 4 qg .. 4 qg + 3 of this row block
+        // (qg = 8 rt + 2 g + lane / 32; S = 2: primes 2 qg, 2 qg + 1 with two
+        // digits each) for entry el form one dword of LO4[qg][el]
         const bool real = s_base[el] >= 0, neg = s_nl[el] < 0;
+        uint32_t *const LO4 = (uint32_t *)LO;
 #pragma unroll
         for (int rt = 0; rt < 2; rt++)
 #pragma unroll
-            for (int g = 0; g < 4; g++)
+            for (int g = 0; g < 4; g++) {
+                uint32_t dw = 0;
 #pragma unroll
                 for (int w = 0; w < 4 / S; w++) {
                     const int ql = (rt * 32 + 8 * g + 4 * (lane >> 5)) / S + w, q = rb * (64 / S) + ql;
                     uint32_t word = 0;
                     if (S == 2 && real && q < P) {
-                        // |acc| <= 2^22 (kb <= 256), so v + off lies in [0, 2^32): 32-bit Barrett
-                        const uint32_t p = s_pc[ql][0], mu32 = s_pc[ql][2];
-                        const uint32_t u = (uint32_t)(acc[rt][4 * g + S * w] + acc[rt][4 * g + S * w + 1] * 256) +
-                                           s_pc[ql][1] + s_pc[ql][3];
-                        uint32_t res = u - __umulhi(u, mu32) * p;   // < 3p
-                        res = res >= p ? res - p : res;
-                        res = res >= p ? res - p : res;
-                        if (neg && res) res = p - res;
-                        word = slice2(centre(res, p));
+                        // |acc| <= 2^22 (kb <= 256): v = acc0 + 256 acc1 + kofs is an int32
+                        // and |v / p| < 2^17, so in binary64 rint(v / p) is the nearest
+                        // quotient (v / p is at least 1/(2p) from a half-integer) and
+                        // v - rint(v / p) p the centred residue; the residue of -x is -r
+                        const uint4 k4 = *(const uint4 *)s_pc[ql];   // 1/p, p, kofs
+                        const int32_t v = acc[rt][4 * g + S * w] + acc[rt][4 * g + S * w + 1] * 256 + (int32_t)k4.w;
+                        const double y = (double)(neg ? -v : v);
+                        const double pinv = __longlong_as_double(((int64_t)k4.y << 32) | k4.x);
+                        word = slice2((int32_t)fma(-rint(y * pinv), (double)k4.z, y));
                     } else if (S == 4 && real && q < P) {
                         const PrimeConst c = pc[q];
                         int64_t v = c.kofs;
@@ -313,18 +342,34 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
                         if (neg && res) res = c.p - res;
                         word = slice_s<S>(centre(res, c.p));
                     }
-#pragma unroll
-                    for (int s = 0; s < S; s++) LO[(S * ql + s) * 128 + el] = (int8_t)(word >> (8 * s));
+                    dw |= word << (16 * w);
                 }
+                LO4[(rt * 8 + 2 * g + (lane >> 5)) * 128 + el] = dw;
+            }
         __syncthreads();
-        for (int c = tid; c < 64 * 8; c += 256) {
-            const int row = c >> 3, part = c & 7;
-            const int64_t grow = (int64_t)rb * 64 + row;   // = S*q + s
-            if (grow < (int64_t)S * P)
-                *(v4i *)(R + grow * plane + e0 + 16 * part) = *(const v4i *)(LO + row * 128 + 16 * part);
+        {
+            // thread: rows 4 qg .. 4 qg + 3 of entries 8 ch .. 8 ch + 7; a
+            // 4 x 8 byte transpose gives each row an 8-byte segment, and the
+            // 16 threads of one qg write 128 consecutive bytes per row
+            const int qg = tid >> 4, ch = tid & 15;
+            const uint4 a = *(const uint4 *)(LO4 + qg * 128 + 8 * ch);
+            const uint4 b = *(const uint4 *)(LO4 + qg * 128 + 8 * ch + 4);
+            const uint32_t d[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int64_t grow = (int64_t)rb * 64 + 4 * qg + k;   // = S*q + s
+                uint32_t lo = 0, hi = 0;
+#pragma unroll
+                for (int x = 0; x < 4; x++) {
+                    lo |= ((d[x] >> (8 * k)) & 255u) << (8 * x);
+                    hi |= ((d[x + 4] >> (8 * k)) & 255u) << (8 * x);
+                }
+                if (grow < (int64_t)S * P) *(uint2 *)(R + grow * plane + e0 + 8 * ch) = make_uint2(lo, hi);
+            }
         }
     }
 #undef CLOAD
+#undef PCLOAD
 }
 
 // ------------------------------------------------------------------------
