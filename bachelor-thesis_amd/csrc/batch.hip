@@ -39,7 +39,7 @@ struct ReconJob {
     int64_t ystride, nent, colsC, Crs, W, cap;
     uint64_t *C;
     int64_t *Wv;
-    uint64_t *G;   // gamma shares, nrb x ystride
+    double *G;     // gamma partial sums, nrb x ystride
 };
 struct PlainJob {
     int64_t K, M, N, Crs, Ars, Brs, W, cap;
@@ -327,7 +327,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             c.pc = v.pc; c.Mc = v.mc; c.M32 = v.m32; c.Y = g.Y; c.P = (int)v.P; c.Kr = (int)v.Kr;
             c.nrb = (int)ceil_div(4 * v.nw, RECON_ROWS); c.nw = (int)v.nw; c.ystride = d.ystride;
             c.nent = p.K * p.N; c.colsC = p.N; c.Crs = p.Crs; c.W = p.cap + 1; c.cap = p.cap; c.C = (uint64_t *)p.C;
-            c.Wv = (int64_t *)(base + o_wv[i]); c.G = (uint64_t *)(base + o_ga[i]);
+            c.Wv = (int64_t *)(base + o_wv[i]); c.G = (double *)(base + o_ga[i]);
             cj[sc].push_back(c);
             cp[sc].push_back(cp[sc].back() + ceil_div(c.nent, RECON_ENT) * c.nrb);
             kj.push_back(c);

@@ -132,7 +132,7 @@ template <int S, bool RESIDENT>
 __global__ __launch_bounds__(256, 2) void crt_recon_gemm_kernel(
     const PrimeConst *__restrict__ pc, int P, const int8_t *__restrict__ Mc, int Kr, int nrb, int nw,
     const uint32_t *__restrict__ Y, int64_t ystride, int64_t nent, int64_t *__restrict__ Wv,
-    uint64_t *__restrict__ G, int64_t total)
+    double *__restrict__ G, int64_t total)
 {
     // XCD-aware order: the row blocks of one entry block share an L2
     int64_t per = (total + 7) / 8;
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256, 2) void crt_recon_gemm_kernel(
 }
 
 __global__ __launch_bounds__(256) void crt_recon_carry_kernel(
-    const int64_t *__restrict__ Wv, int64_t ystride, int nw, int nrb, const uint64_t *__restrict__ G,
+    const int64_t *__restrict__ Wv, int64_t ystride, int nw, int nrb, const double *__restrict__ G,
     const uint32_t *__restrict__ M32, int64_t nent, int64_t colsC, uint64_t *__restrict__ C, int64_t Crs, int64_t W,
     int64_t cap, unsigned *__restrict__ err)
 {
@@ -481,9 +481,9 @@ namespace {
 int recon_row_blocks(const mgx_crt_plan *pl) { return (int)ceil_div(4 * pl->nw, RECON_ROWS); }
 
 // Both reconstruct passes with caller-provided scratch: Wv holds nw x ystride
-// int64 words, G the row blocks' gamma shares, nrb x ystride uint64.
+// int64 words, G the row blocks' partial gamma sums, nrb x ystride binary64.
 int reconstruct_with(const mgx_crt_plan *pl, int64_t K, int64_t N, const uint32_t *Y, mgx_limb *C, int64_t Crs,
-                     int64_t cap, int64_t *Wv, uint64_t *G, hipStream_t s)
+                     int64_t cap, int64_t *Wv, double *G, hipStream_t s)
 {
     const int64_t nent = K * N, ystride = crt_ystride(nent);
     const int nrb = recon_row_blocks(pl);
@@ -521,7 +521,7 @@ int mgx_crt_reconstruct(const mgx_crt_plan *pl, int64_t K, int64_t N, const uint
     char *scr = nullptr;
     MGX_TRY(scratch_alloc((void **)&scr, recon_scratch_bytes(pl, K * N), s));
     int st = reconstruct_with(pl, K, N, Y, C, Crs, cap, (int64_t *)scr,
-                              (uint64_t *)(scr + pl->nw * crt_ystride(K * N) * 8), s);
+                              (double *)(scr + pl->nw * crt_ystride(K * N) * 8), s);
     scratch_free(scr, s);
     return st;
 }
@@ -601,7 +601,7 @@ int mgx_crt_mfma_matmul(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t 
         int8_t *RA = (int8_t *)scr, *RB = (int8_t *)scr + bytesA;
         uint32_t *Y = (uint32_t *)(scr + offY);
         int64_t *Wv = (int64_t *)(scr + offR);
-        uint64_t *gam = (uint64_t *)(scr + offR + pl->nw * crt_ystride(K * N) * 8);
+        double *gam = (double *)(scr + offR + pl->nw * crt_ystride(K * N) * 8);
         st = mgx_crt_reduce(pl, 0, K, M, A, Ars, cap, 0, RA, stream);
         if (st == MGX_OK) st = mgx_crt_reduce(pl, 1, N, M, B, Brs, cap, 0, RB, stream);
         if (st == MGX_OK) st = mgx_crt_residue_gemm(pl, K, M, N, RA, RB, Y, stream);

@@ -746,10 +746,12 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
 //        byte positions) against the digits of yc_q (columns = entries).  The
 //        epilogue folds each group of four byte rows into one signed 32-bit
 //        word value (|value| < 2^49) stored word-major, Wv[t][e].  The
-//        workgroups also form the 32-bit fixed-point sum of y_q / p_q and the
-//        count of negative yc_q (gamma = round(sum) - count, so that
-//        C = X' - gamma M): row block rb takes the K steps ks = rb mod nrb
-//        and writes its share as sum + (count << 50) to G[rb][e].
+//        workgroups also form gamma = round(sum_q yc_q / p_q) = round(X'/M),
+//        so that C = X' - gamma M (|C| <= M/4): row block rb takes the K
+//        steps ks = rb mod nrb and writes its partial sum, a binary64, to
+//        G[rb][e].  Each term is below 1/2 in magnitude, so with at most
+//        2^15 primes the rounding error of the sum stays below 2^-24, far
+//        inside the 1/4 margin.
 //    (b) recon_carry_block: one thread per entry finds the sign of
 //        X' - gamma M from its top words down, then runs the carry chain of
 //        |X' - gamma M| low to high; limbs leave through LDS so that each
@@ -766,7 +768,7 @@ inline int64_t recon_res_lds(int64_t Kr) { return (int64_t)RECON_ROWS * (Kr + 16
 template <int S, bool RESIDENT>
 __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *__restrict__ pc, int P,
     const int8_t *__restrict__ Mc, int Kr, int nrb, int nw, const uint32_t *__restrict__ Y, int64_t ystride,
-    int64_t nent, int64_t *__restrict__ Wv, uint64_t *__restrict__ G)
+    int64_t nent, int64_t *__restrict__ Wv, double *__restrict__ G)
 {
     __shared__ __attribute__((aligned(16))) int8_t LAs[RESIDENT ? 16 : RECON_ROWS * LDS_PITCH];
     extern __shared__ __attribute__((aligned(16))) int8_t LDYN[];
@@ -786,8 +788,7 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
     // register sets that rotate by unrolling the loop by three, so that no
     // copy waits on a load just issued.
     v4i pa[2], y0[2][2], y1[2][2], y2[2][2];
-    uint64_t gfix[2] = {0, 0};
-    int gcnt[2] = {0, 0};
+    double gsum[2] = {0.0, 0.0};
     // A: 128 rows x 64 bytes of Mc, two 16-byte chunks per thread
 #define ALOAD(dst, k0_)                                                                                \
     _Pragma("unroll") for (int u = 0; u < 2; u++) {                                                    \
@@ -823,14 +824,11 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
             _Pragma("unroll") for (int sub = 0; sub < 4 / S; sub++) {                                \
                 const int q = (4 / S) * (k0 / 4 + 8 * kk + 4 * h + i) + sub;                         \
                 if (q < P) {                                                                         \
-                    const uint32_t p = pc[q].p;                                                      \
-                    const uint64_t fx = pc[q].fx;                                                    \
+                    const double pinv = pc[q].pinv;                                                  \
                     _Pragma("unroll") for (int ct = 0; ct < 2; ct++) {                               \
                         const uint32_t wd = (uint32_t)ycur[ct][kk][i];                               \
                         const int32_t yc = S == 4 ? unslice4(wd) : unslice2(wd >> (16 * sub));       \
-                        const uint32_t y = yc < 0 ? (uint32_t)(yc + (int32_t)p) : (uint32_t)yc;      \
-                        gfix[ct] += ((uint64_t)y * fx) >> 31;                                        \
-                        gcnt[ct] += yc < 0;                                                          \
+                        gsum[ct] = fma((double)yc, pinv, gsum[ct]);                                  \
                     }                                                                                \
                 }                                                                                    \
             }                                                                                        \
@@ -878,12 +876,12 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
             }
         }
 #pragma unroll
-    for (int ct = 0; ct < 2; ct++) {
-        const uint32_t lo = __shfl_xor((unsigned)gfix[ct], 32), hi = __shfl_xor((unsigned)(gfix[ct] >> 32), 32);
-        const uint64_t f = gfix[ct] + ((uint64_t)hi << 32 | lo);   // < P 2^32 <= 2^50
-        const uint64_t cnt = (uint64_t)(gcnt[ct] + __shfl_xor(gcnt[ct], 32));
+    for (int ct = 0; ct < 2; ct++) {   // lanes l and l + 32 hold the two halves of an entry's sum
+        const uint64_t b = (uint64_t)__double_as_longlong(gsum[ct]);
+        const uint32_t lo = __shfl_xor((unsigned)b, 32), hi = __shfl_xor((unsigned)(b >> 32), 32);
+        const double f = gsum[ct] + __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
         const int64_t e = e0 + 64 * wave + 32 * ct + c32;
-        if (h == 0 && e < nent) G[(int64_t)rb * ystride + e] = f + (cnt << 50);
+        if (h == 0 && e < nent) G[(int64_t)rb * ystride + e] = f;
     }
 }
 
@@ -891,7 +889,7 @@ constexpr int CARRY_ENT = 256;   // entries per workgroup
 constexpr int CARRY_PITCH = 17;  // limbs per LDS row (16 + 1 padding)
 
 __device__ __forceinline__ void recon_carry_block(int64_t blk, const int64_t *__restrict__ Wv, int64_t ystride,
-    int nw, int nrb, const uint64_t *__restrict__ G, const uint32_t *__restrict__ M32, int64_t nent, int64_t colsC,
+    int nw, int nrb, const double *__restrict__ G, const uint32_t *__restrict__ M32, int64_t nent, int64_t colsC,
     uint64_t *__restrict__ C, int64_t Crs, int64_t W, int64_t cap, unsigned *__restrict__ err)
 {
     __shared__ uint64_t LS[CARRY_ENT * CARRY_PITCH];
@@ -901,14 +899,10 @@ __device__ __forceinline__ void recon_carry_block(int64_t blk, const int64_t *__
     const bool valid = e < nent;
     s_base[tid] = valid ? ((e / colsC) * Crs + e % colsC) * W : -1;
     int64_t gam = 0;
-    if (valid) {   // gamma from the row blocks' shares
-        uint64_t f = 0, cnt = 0;
-        for (int rb = 0; rb < nrb; rb++) {
-            const uint64_t g = G[(int64_t)rb * ystride + e];
-            f += g & ((1ull << 50) - 1);
-            cnt += g >> 50;
-        }
-        gam = (int64_t)((f + (1ull << 31)) >> 32) - (int64_t)cnt;
+    if (valid) {   // gamma from the row blocks' partial sums
+        double f = 0.0;
+        for (int rb = 0; rb < nrb; rb++) f += G[(int64_t)rb * ystride + e];
+        gam = (int64_t)rint(f);
     }
     const int64_t *wp = Wv + (valid ? e : 0);
 
@@ -932,12 +926,17 @@ __device__ __forceinline__ void recon_carry_block(int64_t blk, const int64_t *__
     int64_t top = -1;
     bool hi_nz = false;
     for (int64_t c0 = 0; c0 < nlimb; c0 += 16) {
+        // unconditional loads (word index clamped, padding threads read entry 0)
+        // so that all 32 are in flight at once; masked afterwards
         int64_t xw[32];
 #pragma unroll
         for (int j = 0; j < 32; j++) {
-            const int64_t t = 2 * c0 + j;
-            xw[j] = (valid && t < nw) ? wp[t * ystride] - gam * (int64_t)M32[t] : 0;
+            const int64_t t = min<int64_t>(2 * c0 + j, nw - 1);
+            xw[j] = wp[t * ystride] - gam * (int64_t)M32[t];
         }
+#pragma unroll
+        for (int j = 0; j < 32; j++)
+            if (!valid || 2 * c0 + j >= nw) xw[j] = 0;
 #pragma unroll
         for (int j = 0; j < 16; j++) {
             int64_t x0 = neg ? -xw[2 * j] : xw[2 * j];
