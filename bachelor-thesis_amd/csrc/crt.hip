@@ -487,10 +487,11 @@ int reconstruct_with(const mgx_crt_plan *pl, int64_t K, int64_t N, const uint32_
 {
     const int64_t nent = K * N, ystride = crt_ystride(nent);
     const int nrb = recon_row_blocks(pl);
-    const int64_t total = ceil_div(nent, RECON_ENT) * nrb;
-    if (round_up(total, 8) > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "product too large for one reconstruct launch");
-    const int64_t lds = recon_res_lds(pl->Kr);
+    const int64_t lds = recon_res_lds(pl->Kr, pl->P);
     const bool resident = lds <= RECON_RES_MAX_LDS;
+    // resident workgroups sweep RECON_EPW entry blocks each
+    const int64_t total = ceil_div(nent, RECON_ENT * (resident ? RECON_EPW : 1)) * nrb;
+    if (round_up(total, 8) > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "product too large for one reconstruct launch");
     auto kern = pl->S == 2 ? (resident ? crt_recon_gemm_kernel<2, true> : crt_recon_gemm_kernel<2, false>)
                            : (resident ? crt_recon_gemm_kernel<4, true> : crt_recon_gemm_kernel<4, false>);
     if (resident) MGX_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

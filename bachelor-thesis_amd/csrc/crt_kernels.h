@@ -758,12 +758,14 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
 //        entry's limbs are stored as whole 128-byte lines.
 // ------------------------------------------------------------------------
 constexpr int RECON_ROWS = 128;   // result bytes per workgroup
-constexpr int RECON_ENT = 256;    // entries per workgroup (64 per wave)
-// RESIDENT: the workgroup's whole row block of Mc (128 x Kr bytes) sits in
-// dynamic LDS, loaded once, and the K loop runs without barriers; used while
-// 128 (Kr + 16) bytes fit beside a second workgroup.
+constexpr int RECON_ENT = 256;    // entries per workgroup pass (64 per wave)
+// RESIDENT: the workgroup's whole row block of Mc (128 x Kr bytes) and 1/p of
+// every prime sit in dynamic LDS, loaded once, and the K loop runs without
+// barriers; the workgroup then sweeps RECON_EPW consecutive entry blocks with
+// them.  Used while that fits beside a second workgroup.
 constexpr int RECON_RES_MAX_LDS = 144 * 1024;
-inline int64_t recon_res_lds(int64_t Kr) { return (int64_t)RECON_ROWS * (Kr + 16); }
+constexpr int RECON_EPW = 4;
+inline int64_t recon_res_lds(int64_t Kr, int64_t P) { return (int64_t)RECON_ROWS * (Kr + 16) + 8 * P; }
 
 template <int S, bool RESIDENT>
 __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *__restrict__ pc, int P,
@@ -774,9 +776,34 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
     extern __shared__ __attribute__((aligned(16))) int8_t LDYN[];
     int8_t *const LA = RESIDENT ? LDYN : LAs;
     const int apitch = RESIDENT ? Kr + 16 : LDS_PITCH;
+    double *const s_pinv = (double *)(LDYN + RECON_ROWS * (Kr + 16));   // RESIDENT only
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, h = lane >> 5, c32 = lane & 31;
     const int rb = (int)(blk % nrb);
-    const int64_t e0 = (blk / nrb) * RECON_ENT;
+    const int64_t eb0 = (blk / nrb) * (RESIDENT ? RECON_EPW : 1);
+
+    if (RESIDENT) {
+        // 128 rows x Kr bytes in 16-byte chunks, eight loads in flight per thread
+        const int cpr = Kr / 16, nch = RECON_ROWS * cpr;
+        for (int c0 = 0; c0 < nch; c0 += 8 * 256) {
+            v4i t[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int c = min(c0 + tid + 256 * u, nch - 1), row = c / cpr, part = c - row * cpr;
+                t[u] = *(const v4i *)(Mc + (int64_t)(rb * RECON_ROWS + row) * Kr + 16 * part);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int c = c0 + tid + 256 * u, row = c / cpr, part = c - row * cpr;
+                if (c < nch) *(v4i *)(LA + row * apitch + 16 * part) = t[u];
+            }
+        }
+        for (int q = tid; q < P; q += 256) s_pinv[q] = pc[q].pinv;
+        __syncthreads();
+    }
+
+    for (int64_t eb = eb0; eb < eb0 + (RESIDENT ? RECON_EPW : 1); eb++) {
+    const int64_t e0 = eb * RECON_ENT;
+    if (e0 >= nent) break;
     const uint32_t *yp = Y + e0 + 64 * wave + c32;
 
     v16i acc[4][2];
@@ -824,7 +851,7 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
             _Pragma("unroll") for (int sub = 0; sub < 4 / S; sub++) {                                \
                 const int q = (4 / S) * (k0 / 4 + 8 * kk + 4 * h + i) + sub;                         \
                 if (q < P) {                                                                         \
-                    const double pinv = pc[q].pinv;                                                  \
+                    const double pinv = RESIDENT ? s_pinv[q] : pc[q].pinv;                           \
                     _Pragma("unroll") for (int ct = 0; ct < 2; ct++) {                               \
                         const uint32_t wd = (uint32_t)ycur[ct][kk][i];                               \
                         const int32_t yc = S == 4 ? unslice4(wd) : unslice2(wd >> (16 * sub));       \
@@ -841,16 +868,7 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
         k0 += 64;                                                                                    \
         if (k0 >= Kr) break;                                                                         \
     }
-    if (RESIDENT) {
-        const int cpr = Kr / 16;
-        for (int c = tid; c < RECON_ROWS * cpr; c += 256) {
-            const int row = c / cpr, part = c - row * cpr;
-            *(v4i *)(LA + row * apitch + 16 * part) = *(const v4i *)(Mc + (int64_t)(rb * RECON_ROWS + row) * Kr + 16 * part);
-        }
-        __syncthreads();
-    } else {
-        ALOAD(pa, 0);
-    }
+    if (!RESIDENT) { ALOAD(pa, 0); }
     YLOAD(y0, 0);
     if (64 < Kr) { YLOAD(y1, 64); }
     for (int k0 = 0;;) {
@@ -883,6 +901,7 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
         const int64_t e = e0 + 64 * wave + 32 * ct + c32;
         if (h == 0 && e < nent) G[(int64_t)rb * ystride + e] = f;
     }
+    }   // entry blocks
 }
 
 constexpr int CARRY_ENT = 256;   // entries per workgroup
