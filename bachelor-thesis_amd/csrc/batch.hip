@@ -38,7 +38,8 @@ struct ReconJob {
     int P, Kr, nrb, nw;
     int64_t ystride, nent, colsC, Crs, W, cap;
     uint64_t *C;
-    int64_t *Wv;
+    uint32_t *Wv;  // carried 32-bit words, nw x ystride
+    int32_t *Cb;   // row-block carries, nrb x ystride
     double *G;     // gamma partial sums, nrb x ystride
 };
 struct PlainJob {
@@ -127,7 +128,7 @@ __global__ __launch_bounds__(256, 2) void batch_recon_gemm_kernel(Queue q, const
     int64_t t;
     while (next_item(q, j, t)) {
         const ReconJob &r = jobs[j];
-        recon_gemm_tile<S, false>(t, r.pc, r.P, r.Mc, r.Kr, r.nrb, r.nw, r.Y, r.ystride, r.nent, r.Wv, r.G);
+        recon_gemm_tile<S, false>(t, r.pc, r.P, r.Mc, r.Kr, r.nrb, r.nw, r.Y, r.ystride, r.nent, r.Wv, r.Cb, r.G);
         __syncthreads();
     }
 }
@@ -138,7 +139,8 @@ __global__ __launch_bounds__(256) void batch_recon_carry_kernel(Queue q, const R
     int64_t t;
     while (next_item(q, j, t)) {
         const ReconJob &r = jobs[j];
-        recon_carry_block(t, r.Wv, r.ystride, r.nw, r.nrb, r.G, r.M32, r.nent, r.colsC, r.C, r.Crs, r.W, r.cap, err);
+        recon_carry_block(t, r.Wv, r.Cb, r.ystride, r.nw, r.nrb, r.G, r.M32, r.nent, r.colsC, r.C, r.Crs, r.W, r.cap,
+                          err);
         __syncthreads();
     }
 }
@@ -276,8 +278,8 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
         o_ra[i] = L.take(v.S * v.P * d.rpadA * d.kpad);
         o_rb[i] = L.take(v.S * v.P * d.rpadB * d.kpad);
         o_y[i] = L.take(ceil_div(v.S * v.P, 4) * d.ystride * 4);
-        o_wv[i] = L.take(v.nw * d.ystride * 8);
-        o_ga[i] = L.take(ceil_div(4 * v.nw, RECON_ROWS) * d.ystride * 8);
+        o_wv[i] = L.take(v.nw * d.ystride * 4);
+        o_ga[i] = L.take(ceil_div(4 * v.nw, RECON_ROWS) * d.ystride * 12);   // G (binary64), then Cb (int32)
     }
     const int64_t o_ctr = L.take(64 * 8);
     int64_t o_rj[2], o_rp[2], o_gj[4], o_gp[4], o_cj[2], o_cp[2], o_pj[NCLASS], o_pp[NCLASS];
@@ -327,7 +329,8 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             c.pc = v.pc; c.Mc = v.mc; c.M32 = v.m32; c.Y = g.Y; c.P = (int)v.P; c.Kr = (int)v.Kr;
             c.nrb = (int)ceil_div(4 * v.nw, RECON_ROWS); c.nw = (int)v.nw; c.ystride = d.ystride;
             c.nent = p.K * p.N; c.colsC = p.N; c.Crs = p.Crs; c.W = p.cap + 1; c.cap = p.cap; c.C = (uint64_t *)p.C;
-            c.Wv = (int64_t *)(base + o_wv[i]); c.G = (double *)(base + o_ga[i]);
+            c.Wv = (uint32_t *)(base + o_wv[i]); c.G = (double *)(base + o_ga[i]);
+            c.Cb = (int32_t *)(base + o_ga[i] + c.nrb * d.ystride * 8);
             cj[sc].push_back(c);
             cp[sc].push_back(cp[sc].back() + ceil_div(c.nent, RECON_ENT) * c.nrb);
             kj.push_back(c);

@@ -131,22 +131,23 @@ __global__ __launch_bounds__(128 * WM, 2) void crt_gemm2_kernel(
 template <int S, bool RESIDENT>
 __global__ __launch_bounds__(256, 2) void crt_recon_gemm_kernel(
     const PrimeConst *__restrict__ pc, int P, const int8_t *__restrict__ Mc, int Kr, int nrb, int nw,
-    const uint32_t *__restrict__ Y, int64_t ystride, int64_t nent, int64_t *__restrict__ Wv,
-    double *__restrict__ G, int64_t total)
+    const uint32_t *__restrict__ Y, int64_t ystride, int64_t nent, uint32_t *__restrict__ Wv,
+    int32_t *__restrict__ Cb, double *__restrict__ G, int64_t total)
 {
     // XCD-aware order: the row blocks of one entry block share an L2
     int64_t per = (total + 7) / 8;
     int64_t v = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
     if (v >= total) return;
-    recon_gemm_tile<S, RESIDENT>(v, pc, P, Mc, Kr, nrb, nw, Y, ystride, nent, Wv, G);
+    recon_gemm_tile<S, RESIDENT>(v, pc, P, Mc, Kr, nrb, nw, Y, ystride, nent, Wv, Cb, G);
 }
 
 __global__ __launch_bounds__(256) void crt_recon_carry_kernel(
-    const int64_t *__restrict__ Wv, int64_t ystride, int nw, int nrb, const double *__restrict__ G,
+    const uint32_t *__restrict__ Wv, const int32_t *__restrict__ Cb, int64_t ystride, int nw, int nrb,
+    const double *__restrict__ G,
     const uint32_t *__restrict__ M32, int64_t nent, int64_t colsC, uint64_t *__restrict__ C, int64_t Crs, int64_t W,
     int64_t cap, unsigned *__restrict__ err)
 {
-    recon_carry_block(blockIdx.x, Wv, ystride, nw, nrb, G, M32, nent, colsC, C, Crs, W, cap, err);
+    recon_carry_block(blockIdx.x, Wv, Cb, ystride, nw, nrb, G, M32, nent, colsC, C, Crs, W, cap, err);
 }
 
 }  // namespace mgx
@@ -481,12 +482,16 @@ namespace {
 int recon_row_blocks(const mgx_crt_plan *pl) { return (int)ceil_div(4 * pl->nw, RECON_ROWS); }
 
 // Both reconstruct passes with caller-provided scratch: Wv holds nw x ystride
-// int64 words, G the row blocks' partial gamma sums, nrb x ystride binary64.
+// 32-bit words, then Cb, the row blocks' carries (nrb x ystride int32), and
+// G, their partial gamma sums (nrb x ystride binary64).
 int reconstruct_with(const mgx_crt_plan *pl, int64_t K, int64_t N, const uint32_t *Y, mgx_limb *C, int64_t Crs,
-                     int64_t cap, int64_t *Wv, double *G, hipStream_t s)
+                     int64_t cap, char *scr, hipStream_t s)
 {
     const int64_t nent = K * N, ystride = crt_ystride(nent);
     const int nrb = recon_row_blocks(pl);
+    uint32_t *Wv = (uint32_t *)scr;
+    int32_t *Cb = (int32_t *)(scr + pl->nw * ystride * 4);
+    double *G = (double *)(scr + (pl->nw + nrb) * ystride * 4);
     const int64_t lds = recon_res_lds(pl->Kr, pl->P);
     const bool resident = lds <= RECON_RES_MAX_LDS;
     // resident workgroups sweep RECON_EPW entry blocks each
@@ -496,9 +501,9 @@ int reconstruct_with(const mgx_crt_plan *pl, int64_t K, int64_t N, const uint32_
                            : (resident ? crt_recon_gemm_kernel<4, true> : crt_recon_gemm_kernel<4, false>);
     if (resident) MGX_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(kern, dim3((unsigned)round_up(total, 8)), dim3(256), resident ? (unsigned)lds : 0u, s, pl->d_pc,
-                       (int)pl->P, pl->d_mc, (int)pl->Kr, nrb, (int)pl->nw, Y, ystride, nent, Wv, G, total);
+                       (int)pl->P, pl->d_mc, (int)pl->Kr, nrb, (int)pl->nw, Y, ystride, nent, Wv, Cb, G, total);
     MGX_HIP(hipGetLastError());
-    hipLaunchKernelGGL(crt_recon_carry_kernel, dim3((unsigned)ceil_div(nent, CARRY_ENT)), dim3(256), 0, s, Wv,
+    hipLaunchKernelGGL(crt_recon_carry_kernel, dim3((unsigned)ceil_div(nent, CARRY_ENT)), dim3(256), 0, s, Wv, Cb,
                        ystride, (int)pl->nw, nrb, G, pl->d_m32, nent, N, (uint64_t *)C, Crs, cap + 1, cap,
                        device_error_word());
     MGX_HIP(hipGetLastError());
@@ -507,7 +512,7 @@ int reconstruct_with(const mgx_crt_plan *pl, int64_t K, int64_t N, const uint32_
 
 size_t recon_scratch_bytes(const mgx_crt_plan *pl, int64_t nent)
 {
-    return (size_t)((pl->nw + recon_row_blocks(pl)) * crt_ystride(nent) * 8);
+    return (size_t)((pl->nw * 4 + recon_row_blocks(pl) * 12) * crt_ystride(nent));
 }
 }  // namespace
 
@@ -521,8 +526,7 @@ int mgx_crt_reconstruct(const mgx_crt_plan *pl, int64_t K, int64_t N, const uint
     hipStream_t s = as_stream(stream);
     char *scr = nullptr;
     MGX_TRY(scratch_alloc((void **)&scr, recon_scratch_bytes(pl, K * N), s));
-    int st = reconstruct_with(pl, K, N, Y, C, Crs, cap, (int64_t *)scr,
-                              (double *)(scr + pl->nw * crt_ystride(K * N) * 8), s);
+    int st = reconstruct_with(pl, K, N, Y, C, Crs, cap, scr, s);
     scratch_free(scr, s);
     return st;
 }
@@ -601,12 +605,11 @@ int mgx_crt_mfma_matmul(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t 
     if (st == MGX_OK) {
         int8_t *RA = (int8_t *)scr, *RB = (int8_t *)scr + bytesA;
         uint32_t *Y = (uint32_t *)(scr + offY);
-        int64_t *Wv = (int64_t *)(scr + offR);
-        double *gam = (double *)(scr + offR + pl->nw * crt_ystride(K * N) * 8);
+
         st = mgx_crt_reduce(pl, 0, K, M, A, Ars, cap, 0, RA, stream);
         if (st == MGX_OK) st = mgx_crt_reduce(pl, 1, N, M, B, Brs, cap, 0, RB, stream);
         if (st == MGX_OK) st = mgx_crt_residue_gemm(pl, K, M, N, RA, RB, Y, stream);
-        if (st == MGX_OK) st = reconstruct_with(pl, K, N, Y, C, Crs, cap, Wv, gam, s);
+        if (st == MGX_OK) st = reconstruct_with(pl, K, N, Y, C, Crs, cap, scr + offR, s);
         scratch_free(scr, s);
     }
     if (st == MGX_OK) st = check_device_error(s);

@@ -745,7 +745,10 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
 //        y_q, as an int8 GEMM of the digits of (M/p_q) 256^s (rows = result
 //        byte positions) against the digits of yc_q (columns = entries).  The
 //        epilogue folds each group of four byte rows into one signed 32-bit
-//        word value (|value| < 2^49) stored word-major, Wv[t][e].  The
+//        word value (|value| < 2^49); the row block's 32 words of an entry
+//        then run a carry chain, leaving 32-bit words Wv[t][e] and the
+//        block's carry-out (|c| < 2^18) in Cb[rb][e], which belongs at word
+//        32 (rb + 1), or at word nw for the top block.  The
 //        workgroups also form gamma = round(sum_q yc_q / p_q) = round(X'/M),
 //        so that C = X' - gamma M (|C| <= M/4): row block rb takes the K
 //        steps ks = rb mod nrb and writes its partial sum, a binary64, to
@@ -753,7 +756,8 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
 //        2^15 primes the rounding error of the sum stays below 2^-24, far
 //        inside the 1/4 margin.
 //    (b) recon_carry_block: one thread per entry finds the sign of
-//        X' - gamma M from its top words down, then runs the carry chain of
+//        X' - gamma M from its top words down (the top block's carry first),
+//        then runs the carry chain of
 //        |X' - gamma M| low to high; limbs leave through LDS so that each
 //        entry's limbs are stored as whole 128-byte lines.
 // ------------------------------------------------------------------------
@@ -765,12 +769,13 @@ constexpr int RECON_ENT = 256;    // entries per workgroup pass (64 per wave)
 // them.  Used while that fits beside a second workgroup.
 constexpr int RECON_RES_MAX_LDS = 144 * 1024;
 constexpr int RECON_EPW = 4;
+static_assert(RECON_ROWS / 4 == 32, "the carry pass takes one row block of words per 16-limb chunk");
 inline int64_t recon_res_lds(int64_t Kr, int64_t P) { return (int64_t)RECON_ROWS * (Kr + 16) + 8 * P; }
 
 template <int S, bool RESIDENT>
 __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *__restrict__ pc, int P,
     const int8_t *__restrict__ Mc, int Kr, int nrb, int nw, const uint32_t *__restrict__ Y, int64_t ystride,
-    int64_t nent, int64_t *__restrict__ Wv, double *__restrict__ G)
+    int64_t nent, uint32_t *__restrict__ Wv, int32_t *__restrict__ Cb, double *__restrict__ G)
 {
     __shared__ __attribute__((aligned(16))) int8_t LAs[RESIDENT ? 16 : RECON_ROWS * LDS_PITCH];
     extern __shared__ __attribute__((aligned(16))) int8_t LDYN[];
@@ -879,20 +884,40 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
 #undef RSTEP
 #undef ALOAD
 #undef YLOAD
-    // row 8g + 4h + u of row tile rt is byte u of word rb*32 + rt*8 + 2g + h
+    // Row 8g + 4h + u of row tile rt is byte u of word j = rt*8 + 2g + h of
+    // the block (word rb*32 + j).  Lane c32 + 32h ends up with all 32 words
+    // of entry e0 + 64 wave + 32 h + c32: it keeps the words of that entry it
+    // holds and swaps the others with lane c32 + 32(1 - h).
+    {
+        int64_t wv[32];
 #pragma unroll
-    for (int rt = 0; rt < 4; rt++)
-#pragma unroll
-        for (int ct = 0; ct < 2; ct++) {
-            const int64_t e = e0 + 64 * wave + 32 * ct + c32;
+        for (int rt = 0; rt < 4; rt++)
 #pragma unroll
             for (int g = 0; g < 4; g++) {
-                const int t = rb * (RECON_ROWS / 4) + rt * 8 + 2 * g + h;
-                const int64_t v = (int64_t)acc[rt][ct][4 * g] + ((int64_t)acc[rt][ct][4 * g + 1] << 8) +
-                                  ((int64_t)acc[rt][ct][4 * g + 2] << 16) + ((int64_t)acc[rt][ct][4 * g + 3] << 24);
-                if (t < nw && e < nent) Wv[(int64_t)t * ystride + e] = v;
+                int64_t v[2];
+#pragma unroll
+                for (int ct = 0; ct < 2; ct++)
+                    v[ct] = (int64_t)acc[rt][ct][4 * g] + ((int64_t)acc[rt][ct][4 * g + 1] << 8) +
+                            ((int64_t)acc[rt][ct][4 * g + 2] << 16) + ((int64_t)acc[rt][ct][4 * g + 3] << 24);
+                const int64_t own = h ? v[1] : v[0], give = h ? v[0] : v[1];
+                const uint32_t lo = __shfl_xor((unsigned)give, 32), hi = __shfl_xor((unsigned)((uint64_t)give >> 32), 32);
+                const int64_t got = (int64_t)(((uint64_t)hi << 32) | lo);
+                wv[rt * 8 + 2 * g] = h ? got : own;
+                wv[rt * 8 + 2 * g + 1] = h ? own : got;
+            }
+        const int64_t e = e0 + 64 * wave + 32 * h + c32;
+        const int nv = min(RECON_ROWS / 4, nw - rb * (RECON_ROWS / 4));   // words of this block below nw
+        int64_t carry = 0;
+#pragma unroll
+        for (int j = 0; j < RECON_ROWS / 4; j++) {
+            if (j < nv) {
+                const int64_t x = wv[j] + carry;   // |wv| < 2^49
+                carry = x >> 32;
+                if (e < nent) Wv[(int64_t)(rb * (RECON_ROWS / 4) + j) * ystride + e] = (uint32_t)x;
             }
         }
+        if (e < nent) Cb[(int64_t)rb * ystride + e] = (int32_t)carry;
+    }
 #pragma unroll
     for (int ct = 0; ct < 2; ct++) {   // lanes l and l + 32 hold the two halves of an entry's sum
         const uint64_t b = (uint64_t)__double_as_longlong(gsum[ct]);
@@ -907,8 +932,9 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
 constexpr int CARRY_ENT = 256;   // entries per workgroup
 constexpr int CARRY_PITCH = 17;  // limbs per LDS row (16 + 1 padding)
 
-__device__ __forceinline__ void recon_carry_block(int64_t blk, const int64_t *__restrict__ Wv, int64_t ystride,
-    int nw, int nrb, const double *__restrict__ G, const uint32_t *__restrict__ M32, int64_t nent, int64_t colsC,
+__device__ __forceinline__ void recon_carry_block(int64_t blk, const uint32_t *__restrict__ Wv,
+    const int32_t *__restrict__ Cb, int64_t ystride, int nw, int nrb, const double *__restrict__ G,
+    const uint32_t *__restrict__ M32, int64_t nent, int64_t colsC,
     uint64_t *__restrict__ C, int64_t Crs, int64_t W, int64_t cap, unsigned *__restrict__ err)
 {
     __shared__ uint64_t LS[CARRY_ENT * CARRY_PITCH];
@@ -923,18 +949,23 @@ __device__ __forceinline__ void recon_carry_block(int64_t blk, const int64_t *__
         for (int rb = 0; rb < nrb; rb++) f += G[(int64_t)rb * ystride + e];
         gam = (int64_t)rint(f);
     }
-    const int64_t *wp = Wv + (valid ? e : 0);
+    const uint32_t *wp = Wv + (valid ? e : 0);
+    const int32_t *cp = Cb + (valid ? e : 0);
+    constexpr int BW = RECON_ROWS / 4;   // words per row block
 
-    // Sign: x_t = Wv[t] - gam M32[t] is below 2^56 in magnitude (int32 column
-    // sums of at most 2^17 digit products), so the words below t add less
-    // than 2^25 units of 2^(32t); once the top part H_t reaches 2^28 in
-    // magnitude its sign is the sign of the whole, and below that bound
-    // H_t * 2^32 + x_t cannot overflow.
+    // Sign: X' - gamma M = sum_t x_t 2^(32t) + c_top 2^(32 nw) with
+    // x_t = Wv[t] - gam M32[t] (+ the carry of the block below at a block's
+    // first word), |x_t| < 2^47, so the words below t add less than 2^16
+    // units of 2^(32t); once the top part H_t reaches 2^28 in magnitude its
+    // sign is the sign of the whole, and below that bound H_t * 2^32 + x_t
+    // cannot overflow.
     bool neg = false;
     if (valid) {
-        int64_t H = 0;
+        int64_t H = cp[(int64_t)(nrb - 1) * ystride];
         for (int t = nw - 1; t >= 0; t--) {
-            H = H * 4294967296ll + (wp[(int64_t)t * ystride] - gam * (int64_t)M32[t]);
+            int64_t x = (int64_t)wp[(int64_t)t * ystride] - gam * (int64_t)M32[t];
+            if (t % BW == 0 && t > 0) x += cp[(int64_t)(t / BW - 1) * ystride];
+            H = H * 4294967296ll + x;
             if (H >= (1ll << 28) || H <= -(1ll << 28)) break;
         }
         neg = H < 0;
@@ -951,8 +982,11 @@ __device__ __forceinline__ void recon_carry_block(int64_t blk, const int64_t *__
 #pragma unroll
         for (int j = 0; j < 32; j++) {
             const int64_t t = min<int64_t>(2 * c0 + j, nw - 1);
-            xw[j] = wp[t * ystride] - gam * (int64_t)M32[t];
+            xw[j] = (int64_t)wp[t * ystride] - gam * (int64_t)M32[t];
         }
+        // 2 c0 is a multiple of 32 = BW: word 2 c0 takes the carry of block 2 c0 / BW - 1
+        const int64_t cin = c0 > 0 ? (int64_t)cp[(int64_t)(2 * c0 / BW - 1) * ystride] : 0;
+        xw[0] += cin;
 #pragma unroll
         for (int j = 0; j < 32; j++)
             if (!valid || 2 * c0 + j >= nw) xw[j] = 0;
