@@ -598,23 +598,27 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
         // else B slice (idx - A_DMA) / 8, rows 16 ((idx - A_DMA) % 8).  The
         // 16-row block's start is a wave-uniform base, the lane's row and
         // chunk a 32-bit offset.
+        // g2_swz(16 rb16 + lane / 4) = g(lane) ^ 2 (rb16 & 1), so the lane's
+        // chunk, hence its offset, takes one of two values
         const int8_t *sbase[Sh::DMA_PER_WAVE];
-        uint32_t voff[Sh::DMA_PER_WAVE], dst[Sh::DMA_PER_WAVE];
+        uint32_t dst[Sh::DMA_PER_WAVE];
+        bool odd[Sh::DMA_PER_WAVE];
+        const uint32_t c0 = (lane & 3) ^ (((lane >> 3) ^ (lane >> 5)) & 3);
+        const uint32_t voff_e = (uint32_t)((lane >> 2) * kpad) + 16 * c0, voff_o = voff_e + 16 * (c0 ^ 2) - 16 * c0;
 #pragma unroll
         for (int u = 0; u < Sh::DMA_PER_WAVE; u++) {
             const int idx = wave_u * Sh::DMA_PER_WAVE + u;
             const bool isA = idx < Sh::A_DMA;
             const int j = isA ? idx : idx - Sh::A_DMA, rbn = isA ? Sh::M / 16 : 8;
             const int s = j / rbn, rb16 = j % rbn, rows = isA ? Sh::M : 128;
-            const int row = rb16 * 16 + (lane >> 2), dchunk = (lane & 3) ^ g2_swz(row);
             sbase[u] = uniform_ptr(isA ? gA + s * planeA + (i0 + rb16 * 16) * kpad
                                        : gB + s * planeB + (j0 + rb16 * 16) * kpad);
-            voff[u] = (uint32_t)((lane >> 2) * kpad + 16 * dchunk);
+            odd[u] = rb16 & 1;
             dst[u] = (isA ? 0 : Sh::A_BYTES) + (s * rows + rb16 * 16) * 64;
         }
 #define ISSUE(k_)                                                                                     \
         _Pragma("unroll") for (int u = 0; u < Sh::DMA_PER_WAVE; u++)                                  \
-            dma16(sbase[u] + (int64_t)(k_) * 64, voff[u],                                              \
+            dma16(sbase[u] + (int64_t)(k_) * 64, odd[u] ? voff_o : voff_e,                             \
                   __builtin_amdgcn_readfirstlane(ls0 + ((k_) % STAGES) * Sh::STAGE + dst[u]));
 
         v16i acc[3][2][2];
@@ -679,24 +683,30 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
 #pragma unroll
             for (int kk = 0; kk < 2; kk++) {
                 const int ch = 2 * kk + h;
-                v4i af[2][2], bf[2][2];
+                v4i bf[2][2];
 #pragma unroll
-                for (int s = 0; s < 2; s++)
+                for (int u = 0; u < 2; u++)
 #pragma unroll
                     for (int x = 0; x < 2; x++) {
-                        const int ra = wi * 64 + x * 32 + l32, rb = wj * 64 + x * 32 + l32;
-                        af[s][x] = *(const v4i *)(sa + (s * Sh::M + ra) * 64 + 16 * (ch ^ g2_swz(ra)));
-                        bf[s][x] = *(const v4i *)(sb + (s * 128 + rb) * 64 + 16 * (ch ^ g2_swz(rb)));
+                        const int rb = wj * 64 + x * 32 + l32;
+                        bf[u][x] = *(const v4i *)(sb + (u * 128 + rb) * 64 + 16 * (ch ^ g2_swz(rb)));
                     }
 #pragma unroll
-                for (int s = 0; s < 2; s++)
+                for (int s = 0; s < 2; s++) {   // one slice of A at a time: fewer live fragments
+                    v4i af[2];
+#pragma unroll
+                    for (int x = 0; x < 2; x++) {
+                        const int ra = wi * 64 + x * 32 + l32;
+                        af[x] = *(const v4i *)(sa + (s * Sh::M + ra) * 64 + 16 * (ch ^ g2_swz(ra)));
+                    }
 #pragma unroll
                     for (int u = 0; u < 2; u++)
 #pragma unroll
                         for (int rt = 0; rt < 2; rt++)
 #pragma unroll
                             for (int ct = 0; ct < 2; ct++)
-                                acc[s + u][rt][ct] = mfma32(af[s][rt], bf[u][ct], acc[s + u][rt][ct]);
+                                acc[s + u][rt][ct] = mfma32(af[rt], bf[u][ct], acc[s + u][rt][ct]);
+                }
             }
             since += 64;
             if (CHUNKED && since == KCHUNK2 && k + 1 < nsteps) {
