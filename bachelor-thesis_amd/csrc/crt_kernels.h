@@ -323,16 +323,19 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
                 for (int w = 0; w < 4 / S; w++) {
                     const int ql = (rt * 32 + 8 * g + 4 * (lane >> 5)) / S + w, q = rb * (64 / S) + ql;
                     uint32_t word = 0;
-                    if (S == 2 && real && q < P) {
+                    if (S == 2) {
                         // |acc| <= 2^22 (kb <= 256): v = acc0 + 256 acc1 + kofs is an int32
                         // and |v / p| < 2^17, so in binary64 rint(v / p) is the nearest
                         // quotient (v / p is at least 1/(2p) from a half-integer) and
-                        // v - rint(v / p) p the centred residue; the residue of -x is -r
+                        // v - rint(v / p) p the centred residue; the residue of -x is -r.
+                        // Computed for every lane and selected afterwards, so that the
+                        // constants' LDS reads are not each behind a branch.
                         const uint4 k4 = *(const uint4 *)s_pc[ql];   // 1/p, p, kofs
                         const int32_t v = acc[rt][4 * g + S * w] + acc[rt][4 * g + S * w + 1] * 256 + (int32_t)k4.w;
                         const double y = (double)(neg ? -v : v);
                         const double pinv = __longlong_as_double(((int64_t)k4.y << 32) | k4.x);
-                        word = slice2((int32_t)fma(-rint(y * pinv), (double)k4.z, y));
+                        const uint32_t r = slice2((int32_t)fma(-rint(y * pinv), (double)k4.z, y));
+                        word = (real && q < P) ? r : 0u;
                     } else if (S == 4 && real && q < P) {
                         const PrimeConst c = pc[q];
                         int64_t v = c.kofs;
