@@ -899,24 +899,27 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
 #undef YLOAD
     // Row 8g + 4h + u of row tile rt is byte u of word j = rt*8 + 2g + h of
     // the block (word rb*32 + j).  Lane c32 + 32h ends up with all 32 words
-    // of entry e0 + 64 wave + 32 h + c32: it keeps the words of that entry it
-    // holds and swaps the others with lane c32 + 32(1 - h).
+    // of entry e0 + 64 wave + 32 h + c32: v_permlane32_swap exchanges the
+    // upper half-wave of the ct = 0 values with the lower half-wave of the
+    // ct = 1 values, after which the first operand holds word 2g' and the
+    // second word 2g' + 1 of the lane's own entry.
     {
         int64_t wv[32];
 #pragma unroll
         for (int rt = 0; rt < 4; rt++)
 #pragma unroll
             for (int g = 0; g < 4; g++) {
-                int64_t v[2];
+                uint64_t v[2];
 #pragma unroll
                 for (int ct = 0; ct < 2; ct++)
-                    v[ct] = (int64_t)acc[rt][ct][4 * g] + ((int64_t)acc[rt][ct][4 * g + 1] << 8) +
-                            ((int64_t)acc[rt][ct][4 * g + 2] << 16) + ((int64_t)acc[rt][ct][4 * g + 3] << 24);
-                const int64_t own = h ? v[1] : v[0], give = h ? v[0] : v[1];
-                const uint32_t lo = __shfl_xor((unsigned)give, 32), hi = __shfl_xor((unsigned)((uint64_t)give >> 32), 32);
-                const int64_t got = (int64_t)(((uint64_t)hi << 32) | lo);
-                wv[rt * 8 + 2 * g] = h ? got : own;
-                wv[rt * 8 + 2 * g + 1] = h ? own : got;
+                    v[ct] = (uint64_t)((int64_t)acc[rt][ct][4 * g] + ((int64_t)acc[rt][ct][4 * g + 1] << 8) +
+                                       ((int64_t)acc[rt][ct][4 * g + 2] << 16) +
+                                       ((int64_t)acc[rt][ct][4 * g + 3] << 24));
+                const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)v[0], (unsigned)v[1], false, false);
+                const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(v[0] >> 32), (unsigned)(v[1] >> 32),
+                                                                 false, false);
+                wv[rt * 8 + 2 * g] = (int64_t)(((uint64_t)hi[0] << 32) | lo[0]);
+                wv[rt * 8 + 2 * g + 1] = (int64_t)(((uint64_t)hi[1] << 32) | lo[1]);
             }
         const int64_t e = e0 + 64 * wave + 32 * h + c32;
         const int nv = min(RECON_ROWS / 4, nw - rb * (RECON_ROWS / 4));   // words of this block below nw
