@@ -782,6 +782,14 @@ constexpr int RECON_ENT = 256;    // entries per workgroup pass (64 per wave)
 // them.  Used while that fits beside a second workgroup.
 constexpr int RECON_RES_MAX_LDS = 144 * 1024;
 constexpr int RECON_EPW = 4;
+// Wv, Cb and G are stored in blocks of RECON_ENT entries: index t of entry e
+// (t < nt) at (e / RECON_ENT) nt RECON_ENT + t RECON_ENT + e % RECON_ENT, so
+// that a reconstruct workgroup's words and the carry pass's reads are long
+// contiguous runs.
+__device__ __forceinline__ int64_t recon_idx(int64_t t, int64_t e, int64_t nt)
+{
+    return (e / RECON_ENT) * nt * RECON_ENT + t * RECON_ENT + e % RECON_ENT;
+}
 static_assert(RECON_ROWS / 4 == 32, "the carry pass takes one row block of words per 16-limb chunk");
 inline int64_t recon_res_lds(int64_t Kr, int64_t P) { return (int64_t)RECON_ROWS * (Kr + 16) + 8 * P; }
 
@@ -929,10 +937,10 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
             if (j < nv) {
                 const int64_t x = wv[j] + carry;   // |wv| < 2^49
                 carry = x >> 32;
-                if (e < nent) Wv[(int64_t)(rb * (RECON_ROWS / 4) + j) * ystride + e] = (uint32_t)x;
+                if (e < nent) Wv[recon_idx(rb * (RECON_ROWS / 4) + j, e, nw)] = (uint32_t)x;
             }
         }
-        if (e < nent) Cb[(int64_t)rb * ystride + e] = (int32_t)carry;
+        if (e < nent) Cb[recon_idx(rb, e, nrb)] = (int32_t)carry;
     }
 #pragma unroll
     for (int ct = 0; ct < 2; ct++) {   // lanes l and l + 32 hold the two halves of an entry's sum
@@ -940,7 +948,7 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
         const uint32_t lo = __shfl_xor((unsigned)b, 32), hi = __shfl_xor((unsigned)(b >> 32), 32);
         const double f = gsum[ct] + __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
         const int64_t e = e0 + 64 * wave + 32 * ct + c32;
-        if (h == 0 && e < nent) G[(int64_t)rb * ystride + e] = f;
+        if (h == 0 && e < nent) G[recon_idx(rb, e, nrb)] = f;
     }
     }   // entry blocks
 }
@@ -962,11 +970,12 @@ __device__ __forceinline__ void recon_carry_block(int64_t blk, const uint32_t *_
     int64_t gam = 0;
     if (valid) {   // gamma from the row blocks' partial sums
         double f = 0.0;
-        for (int rb = 0; rb < nrb; rb++) f += G[(int64_t)rb * ystride + e];
+        for (int rb = 0; rb < nrb; rb++) f += G[recon_idx(rb, e, nrb)];
         gam = (int64_t)rint(f);
     }
-    const uint32_t *wp = Wv + (valid ? e : 0);
-    const int32_t *cp = Cb + (valid ? e : 0);
+    // this entry's words and block carries: stride 256 within its entry block
+    const uint32_t *wp = Wv + (valid ? recon_idx(0, e, nw) : 0);
+    const int32_t *cp = Cb + (valid ? recon_idx(0, e, nrb) : 0);
     constexpr int BW = RECON_ROWS / 4;   // words per row block
 
     // Sign: X' - gamma M = sum_t x_t 2^(32t) + c_top 2^(32 nw) with
@@ -977,10 +986,10 @@ __device__ __forceinline__ void recon_carry_block(int64_t blk, const uint32_t *_
     // cannot overflow.
     bool neg = false;
     if (valid) {
-        int64_t H = cp[(int64_t)(nrb - 1) * ystride];
+        int64_t H = cp[(int64_t)(nrb - 1) * RECON_ENT];
         for (int t = nw - 1; t >= 0; t--) {
-            int64_t x = (int64_t)wp[(int64_t)t * ystride] - gam * (int64_t)M32[t];
-            if (t % BW == 0 && t > 0) x += cp[(int64_t)(t / BW - 1) * ystride];
+            int64_t x = (int64_t)wp[(int64_t)t * RECON_ENT] - gam * (int64_t)M32[t];
+            if (t % BW == 0 && t > 0) x += cp[(int64_t)(t / BW - 1) * RECON_ENT];
             H = H * 4294967296ll + x;
             if (H >= (1ll << 28) || H <= -(1ll << 28)) break;
         }
@@ -998,10 +1007,10 @@ __device__ __forceinline__ void recon_carry_block(int64_t blk, const uint32_t *_
 #pragma unroll
         for (int j = 0; j < 32; j++) {
             const int64_t t = min<int64_t>(2 * c0 + j, nw - 1);
-            xw[j] = (int64_t)wp[t * ystride] - gam * (int64_t)M32[t];
+            xw[j] = (int64_t)wp[t * RECON_ENT] - gam * (int64_t)M32[t];
         }
         // 2 c0 is a multiple of 32 = BW: word 2 c0 takes the carry of block 2 c0 / BW - 1
-        const int64_t cin = c0 > 0 ? (int64_t)cp[(int64_t)(2 * c0 / BW - 1) * ystride] : 0;
+        const int64_t cin = c0 > 0 ? (int64_t)cp[(int64_t)(2 * c0 / BW - 1) * RECON_ENT] : 0;
         xw[0] += cin;
 #pragma unroll
         for (int j = 0; j < 32; j++)
