@@ -100,6 +100,15 @@ __device__ __forceinline__ v4i mfma16(v4i a, v4i b, v4i c)
 constexpr int LDS_PITCH = 80;   // 64 data bytes + 16 padding per row
 
 
+// Byte offset of residue (row r, column k) in an S = 2 residue plane: blocks
+// of 16 rows x 64 columns (1 KB, row-major inside), the blocks of a 16-row band
+// in column order, so that the residue GEMM's 16-row x 64-byte loads are
+// whole contiguous kilobytes.  S = 4 planes are plain row-major.
+__host__ __device__ __forceinline__ int64_t res_off(int S, int64_t r, int64_t k, int64_t kpad)
+{
+    return S == 2 ? (r >> 4) * (kpad << 4) + ((k >> 6) << 10) + ((r & 15) << 6) + (k & 63) : r * kpad + k;
+}
+
 // ------------------------------------------------------------------------
 // 1. reduce.  Output rows S*q+s x entries; tile 64 rows x 128 entries.
 // ------------------------------------------------------------------------
@@ -191,7 +200,7 @@ __device__ __forceinline__ void reduce_tile(int64_t blk,
                     if (neg && res) res = c.p - res;
                     word = slice_s<S>(centre(res, c.p));
                 }
-                int8_t *o = R + (int64_t)q * S * plane + e;
+                int8_t *o = R + (int64_t)q * S * plane + res_off(S, e / kpad, e % kpad, kpad);
 #pragma unroll
                 for (int s = 0; s < S; s++) o[s * plane] = (int8_t)(word >> (8 * s));
             }
@@ -355,6 +364,7 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
             // 4 x 8 byte transpose gives each row an 8-byte segment, and the
             // 16 threads of one qg write 128 consecutive bytes per row
             const int qg = tid >> 4, ch = tid & 15;
+            const int64_t eo = res_off(S, (e0 + 8 * ch) / kpad, (e0 + 8 * ch) % kpad, kpad);   // 8 columns, one row
             const uint4 a = *(const uint4 *)(LO4 + qg * 128 + 8 * ch);
             const uint4 b = *(const uint4 *)(LO4 + qg * 128 + 8 * ch + 4);
             const uint32_t d[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
@@ -367,7 +377,7 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
                     lo |= ((d[x] >> (8 * k)) & 255u) << (8 * x);
                     hi |= ((d[x + 4] >> (8 * k)) & 255u) << (8 * x);
                 }
-                if (grow < (int64_t)S * P) *(uint2 *)(R + grow * plane + e0 + 8 * ch) = make_uint2(lo, hi);
+                if (grow < (int64_t)S * P) *(uint2 *)(R + grow * plane + eo) = make_uint2(lo, hi);
             }
         }
     }
@@ -607,7 +617,8 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
         uint32_t dst[Sh::DMA_PER_WAVE];
         bool odd[Sh::DMA_PER_WAVE];
         const uint32_t c0 = (lane & 3) ^ (((lane >> 3) ^ (lane >> 5)) & 3);
-        const uint32_t voff_e = (uint32_t)((lane >> 2) * kpad) + 16 * c0, voff_o = voff_e + 16 * (c0 ^ 2) - 16 * c0;
+        // (residue planes in 16-row x 64-byte blocks: row l/4 of a block is 64 bytes in)
+        const uint32_t voff_e = (uint32_t)((lane >> 2) * 64) + 16 * c0, voff_o = voff_e + 16 * (c0 ^ 2) - 16 * c0;
 #pragma unroll
         for (int u = 0; u < Sh::DMA_PER_WAVE; u++) {
             const int idx = wave_u * Sh::DMA_PER_WAVE + u;
@@ -621,7 +632,7 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
         }
 #define ISSUE(k_)                                                                                     \
         _Pragma("unroll") for (int u = 0; u < Sh::DMA_PER_WAVE; u++)                                  \
-            dma16(sbase[u] + (int64_t)(k_) * 64, odd[u] ? voff_o : voff_e,                             \
+            dma16(sbase[u] + (int64_t)(k_) * 1024, odd[u] ? voff_o : voff_e,                           \
                   __builtin_amdgcn_readfirstlane(ls0 + ((k_) % STAGES) * Sh::STAGE + dst[u]));
 
         v16i acc[3][2][2];

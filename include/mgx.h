@@ -199,8 +199,10 @@ int mgx_crt_mfma_matmul(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t 
                         const mgx_limb *A, int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap,
                         void *stream);
 /* Stage kernels, exposed for testing and profiling.  Residues are planar
- * int8 slices R[q][s][row][k] (s < S) with row pitch kpad and rows padded to
- * rpad (mgx_crt_residue_dims).  Y holds y_q = (C_ij mod p_q) (M/p_q)^-1 mod
+ * int8 slices R[q][s] (s < S) of rpad x kpad bytes (mgx_crt_residue_dims):
+ * row-major for S = 4; for S = 2 in blocks of 16 rows x 64 columns (1 KB,
+ * row-major inside), the blocks of each 16-row band in column order, i.e.
+ * (row, k) at (row/16)*16*kpad + (k/64)*1024 + (row%16)*64 + k%64.  Y holds y_q = (C_ij mod p_q) (M/p_q)^-1 mod
  * p_q, centred and written as S balanced int8 digits, at entry i*N+j of
  * uint32 word planes of ystride = mgx_crt_y_stride(K, N) words (K*N rounded
  * up to 256): plane q for S = 4 (byte s = digit s); for S = 2 plane q/2,

@@ -59,6 +59,9 @@ def test_reduce_kernel(gpu, slices, transpose):
         mgx.call("mgx_crt_reduce", plan, transpose, rows, K, dX.ptr, xrs, cap, 0, dR.ptr, None)
         mgx.call("mgx_check_error", None)
         R = dR.to_array((len(ps), S, rpad, kpad), dtype=np.int8)
+        if S == 2:   # planes in 16-row x 64-column blocks, a band's blocks in column order
+            R = R.reshape(len(ps), S, rpad // 16, kpad // 64, 16, 64).transpose(0, 1, 2, 4, 3, 5).reshape(
+                len(ps), S, rpad, kpad)
         for qi in [0, 1, len(ps) // 2, len(ps) - 1]:
             p = ps[qi]
             for r in range(rows):
