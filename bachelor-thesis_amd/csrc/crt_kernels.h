@@ -207,10 +207,11 @@ __device__ __forceinline__ void reduce_tile(int64_t blk,
 }
 
 
-// Entry-resident reduce for entries of at most 256 bytes: one workgroup
-// stages its 128 entries in LDS once and sweeps every prime block over them;
-// the residues of a prime block leave through LDS as 16-byte stores of 128
-// consecutive entries per (prime, slice) plane.
+// Entry-resident reduce for entries of at most 256 bytes: every lane keeps
+// half of one entry's bytes in registers for the whole sweep (they are the
+// MFMA B operand of every prime block), and the workgroup sweeps every prime
+// block over its 128 entries; the residues of a prime block leave through
+// LDS as 8-byte row segments.
 constexpr int RES_KB = 256;
 constexpr int RES_PITCH = RES_KB + 16;
 
@@ -220,57 +221,44 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
     const uint64_t *__restrict__ X, int64_t Xrs, int64_t W, int transpose, int64_t rows, int64_t K,
     int64_t kpad, int8_t *__restrict__ R, int64_t plane, unsigned *__restrict__ err)
 {
-    __shared__ __attribute__((aligned(16))) int8_t LE[128 * RES_PITCH];
     __shared__ __attribute__((aligned(16))) int8_t LC[64 * RES_PITCH];
     __shared__ __attribute__((aligned(16))) int8_t LO[64 * 128];
-    __shared__ int64_t s_base[128];   // entry offset in words, -1 for padding
-    __shared__ int s_nl[128];         // limbs to read, ~limbs when negative
     __shared__ __attribute__((aligned(16))) uint32_t s_pc[32][4];   // S = 2: 1/p (two words), p, kofs
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t e0 = blk * 128;
     const int cpe = kb / 16;   // 16-byte chunks per entry
+    const int el = wave * 32 + (lane & 31), hh = lane >> 5;
 
-    if (tid < 128) {
-        const int64_t e = e0 + tid, r = e / kpad, k = e % kpad;
-        s_base[tid] = (r < rows && k < K) ? (transpose ? (k * Xrs + r) * W : (r * Xrs + k) * W) : -1;
-    }
-    __syncthreads();
-    // thread c of the 128 x cpe chunks (at most 8 per thread) reads its entry's
-    // size word and the chunk's two limbs with no condition, so that all the
-    // loads are in flight at once (padding reads entry 0, limb indices are
-    // clamped below the entry's cap); what lies at or above |size| is masked
-    // afterwards
+    // This lane's entry, bytes 32 k32 + 16 hh .. + 15 for every k32 < kb/32
+    // (limbs 4 k32 + 2 hh, + 1): loaded with no condition, all in flight at
+    // once (padding reads entry 0, limb indices clamped below the cap), then
+    // masked at |size| and biased to byte - 128.
+    bool real, neg;
+    v4i bf[RES_KB / 32];
     {
+        const int64_t e = e0 + el, r = e / kpad, k = e % kpad;
+        real = r < rows && k < K;
+        const int64_t base = real ? (transpose ? (k * Xrs + r) * W : (r * Xrs + k) * W) : 0;
         const int lim = (int)min<int64_t>(kb / 8, W - 1);
-        uint64_t v[8][2];
-        int64_t sz[8], bs[8];
+        const int64_t sz = (int64_t)X[base + W - 1];
+        uint64_t v[RES_KB / 32][2];
 #pragma unroll
-        for (int it = 0; it < 8; it++) {
-            const int c = tid + 256 * it, el = min(c / cpe, 127), part = c - (c / cpe) * cpe;
-            bs[it] = c < 128 * cpe ? s_base[el] : -1;
-            const int64_t b0 = bs[it] >= 0 ? bs[it] : 0;
-            sz[it] = (int64_t)X[b0 + W - 1];
-            v[it][0] = X[b0 + min(2 * part, lim - 1)];
-            v[it][1] = X[b0 + min(2 * part + 1, lim - 1)];
-        }
+        for (int k32 = 0; k32 < RES_KB / 32; k32++)
 #pragma unroll
-        for (int it = 0; it < 8; it++) {
-            const int c = tid + 256 * it, el = c / cpe, part = c - el * cpe;
-            if (c < 128 * cpe) {
-                const int64_t a = bs[it] >= 0 ? min(sz[it] < 0 ? -sz[it] : sz[it], W - 1) : 0;
-                const int nl = (int)min<int64_t>(a, lim);
-                uint64_t *d = (uint64_t *)(LE + el * RES_PITCH + 16 * part);
-                d[0] = (2 * part < nl ? v[it][0] : 0) ^ 0x8080808080808080ull;   // byte - 128
-                d[1] = (2 * part + 1 < nl ? v[it][1] : 0) ^ 0x8080808080808080ull;
-                if (part == 0) {
-                    s_nl[el] = (bs[it] >= 0 && sz[it] < 0) ? ~nl : nl;
-                    for (int64_t x = kb / 8; x < a; x++)   // limbs beyond the plan's bound must be zero
-                        if (X[bs[it] + x]) { atomicOr(err, DEV_ERR_INPUT); break; }
-                }
-            }
+            for (int u = 0; u < 2; u++) v[k32][u] = X[base + min(4 * k32 + 2 * hh + u, lim - 1)];
+        const int64_t a = real ? min(sz < 0 ? -sz : sz, W - 1) : 0;
+        const int nl = (int)min<int64_t>(a, lim);
+        neg = real && sz < 0;
+#pragma unroll
+        for (int k32 = 0; k32 < RES_KB / 32; k32++) {
+            const uint64_t d0 = (4 * k32 + 2 * hh < nl ? v[k32][0] : 0) ^ 0x8080808080808080ull;
+            const uint64_t d1 = (4 * k32 + 2 * hh + 1 < nl ? v[k32][1] : 0) ^ 0x8080808080808080ull;
+            bf[k32] = (v4i){(int)(uint32_t)d0, (int)(uint32_t)(d0 >> 32), (int)(uint32_t)d1, (int)(uint32_t)(d1 >> 32)};
         }
+        if (hh == 0)
+            for (int64_t x = kb / 8; x < a; x++)   // limbs beyond the plan's bound must be zero
+                if (X[base + x]) { atomicOr(err, DEV_ERR_INPUT); break; }
     }
-    const int el = wave * 32 + (lane & 31);
     // the constants of row block rb: 64 rows x kb bytes, at most four 16-byte
     // chunks per thread, one block ahead in registers
     v4i cpre[4];
@@ -295,7 +283,7 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
     CLOAD(0);
     PCLOAD(0);
     for (int rb = 0; rb < nrb; rb++) {
-        __syncthreads();   // LE staged; LC, LO and s_pc free again
+        __syncthreads();   // LC, LO and s_pc free again
 #pragma unroll
         for (int u = 0; u < 4; u++)
             if (cst_off[u] >= 0) *(v4i *)(LC + lc_off[u]) = cpre[u];
@@ -309,19 +297,17 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
 #pragma unroll
         for (int k32 = 0; k32 < RES_KB / 32; k32++) {
             if (k32 * 32 < kb) {
-                const int ko = k32 * 32 + 16 * (lane >> 5);
-                const v4i bf = *(const v4i *)(LE + el * RES_PITCH + ko);
+                const int ko = k32 * 32 + 16 * hh;
 #pragma unroll
                 for (int rt = 0; rt < 2; rt++) {
                     const v4i af = *(const v4i *)(LC + (rt * 32 + (lane & 31)) * RES_PITCH + ko);
-                    acc[rt] = mfma32(af, bf, k32 == 0 ? (v16i){0} : acc[rt]);
+                    acc[rt] = mfma32(af, bf[k32], k32 == 0 ? (v16i){0} : acc[rt]);
                 }
             }
         }
         // the residue digits of rows 4 qg .. 4 qg + 3 of this row block
         // (qg = 8 rt + 2 g + lane / 32; S = 2: primes 2 qg, 2 qg + 1 with two
         // digits each) for entry el form one dword of LO4[qg][el]
-        const bool real = s_base[el] >= 0, neg = s_nl[el] < 0;
         uint32_t *const LO4 = (uint32_t *)LO;
 #pragma unroll
         for (int rt = 0; rt < 2; rt++)
