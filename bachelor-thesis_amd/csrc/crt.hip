@@ -42,13 +42,24 @@ namespace mgx {
 // read; each thread has the loads of four entries in flight at once (the scan
 // is latency-bound: two dependent 8-byte loads per 536-byte entry).
 // ------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void scan_bits_kernel(const uint64_t *X, int64_t rs, int64_t W, int64_t rows,
-                                                        int64_t cols, int *out)
+// Up to two matrices in one launch (workgroups [0, nblk0) scan the first).
+struct ScanArgs {
+    const uint64_t *X[2];
+    int64_t rs[2], rows[2], cols[2];
+    int nblk0;
+};
+
+__global__ __launch_bounds__(256) void scan_bits_kernel(ScanArgs sa, int64_t W, int *out_pair)
 {
     __shared__ int s_best[4];
-    const int64_t n = rows * cols, stride = (int64_t)gridDim.x * blockDim.x;
+    const int m = (int)blockIdx.x < sa.nblk0 ? 0 : 1;
+    const int64_t blk = m ? blockIdx.x - sa.nblk0 : blockIdx.x, nblk = m ? gridDim.x - sa.nblk0 : sa.nblk0;
+    const uint64_t *X = sa.X[m];
+    const int64_t rs = sa.rs[m], cols = sa.cols[m];
+    int *out = out_pair + m;
+    const int64_t n = sa.rows[m] * cols, stride = nblk * blockDim.x;
     int best = 0;
-    for (int64_t e0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e0 < n; e0 += 4 * stride) {
+    for (int64_t e0 = blk * (int64_t)blockDim.x + threadIdx.x; e0 < n; e0 += 4 * stride) {
         const uint64_t *p[4];
         int64_t a[4];
         uint64_t top[4];
@@ -543,13 +554,19 @@ int scan_bits_pair(int nmat, const int64_t *rows, const int64_t *cols, const mgx
     int *d = nullptr;
     MGX_TRY(scratch_alloc((void **)&d, 2 * sizeof(int), s));
     MGX_HIP(hipMemsetAsync(d, 0, 2 * sizeof(int), s));
+    ScanArgs sa{};
+    int grid[2] = {0, 0};
     for (int m = 0; m < nmat; m++) {
-        int64_t n = rows[m] * cols[m];
-        int grid = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(n, 1024), 1), 2048);
-        if (n > 0)
-            hipLaunchKernelGGL(scan_bits_kernel, dim3(grid), dim3(256), 0, s, (const uint64_t *)X[m], Xrs[m], cap + 1,
-                               rows[m], cols[m], d + m);
+        const int64_t n = rows[m] * cols[m];
+        sa.X[m] = (const uint64_t *)X[m];
+        sa.rs[m] = Xrs[m];
+        sa.rows[m] = rows[m];
+        sa.cols[m] = cols[m];
+        grid[m] = n > 0 ? (int)std::min<int64_t>(ceil_div(n, 1024), 2048) : 0;
     }
+    sa.nblk0 = grid[0];
+    if (grid[0] + grid[1] > 0)
+        hipLaunchKernelGGL(scan_bits_kernel, dim3(grid[0] + grid[1]), dim3(256), 0, s, sa, cap + 1, d);
     int h[2] = {0, 0};
     MGX_HIP(hipMemcpyAsync(h, d, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
     MGX_TRY(scratch_free(d, s));
