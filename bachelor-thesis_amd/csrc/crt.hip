@@ -503,7 +503,7 @@ int reconstruct_with(const mgx_crt_plan *pl, int64_t K, int64_t N, const uint32_
     uint32_t *Wv = (uint32_t *)scr;
     int32_t *Cb = (int32_t *)(scr + pl->nw * ystride * 4);
     double *G = (double *)(scr + (pl->nw + nrb) * ystride * 4);
-    const int64_t lds = recon_res_lds(pl->Kr, pl->P);
+    const int64_t lds = recon_res_lds(pl->Kr, pl->S);
     const bool resident = lds <= RECON_RES_MAX_LDS;
     // resident workgroups sweep RECON_EPW entry blocks each
     const int64_t total = ceil_div(nent, RECON_ENT * (resident ? RECON_EPW : 1)) * nrb;

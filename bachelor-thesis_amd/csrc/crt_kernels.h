@@ -788,7 +788,8 @@ __device__ __forceinline__ int64_t recon_idx(int64_t t, int64_t e, int64_t nt)
     return (e / RECON_ENT) * nt * RECON_ENT + t * RECON_ENT + e % RECON_ENT;
 }
 static_assert(RECON_ROWS / 4 == 32, "the carry pass takes one row block of words per 16-limb chunk");
-inline int64_t recon_res_lds(int64_t Kr, int64_t P) { return (int64_t)RECON_ROWS * (Kr + 16) + 8 * P; }
+// (1/p for every digit row q < Kr/S, zero past the last prime)
+inline int64_t recon_res_lds(int64_t Kr, int64_t S) { return (int64_t)RECON_ROWS * (Kr + 16) + 8 * (Kr / S); }
 
 template <int S, bool RESIDENT>
 __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *__restrict__ pc, int P,
@@ -820,7 +821,7 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
                 if (c < nch) *(v4i *)(LA + row * apitch + 16 * part) = t[u];
             }
         }
-        for (int q = tid; q < P; q += 256) s_pinv[q] = pc[q].pinv;
+        for (int q = tid; q < Kr / S; q += 256) s_pinv[q] = q < P ? pc[q].pinv : 0.0;
         __syncthreads();
     }
 
@@ -873,13 +874,12 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
             _Pragma("unroll") for (int i = 0; i < 4; i++)                                            \
             _Pragma("unroll") for (int sub = 0; sub < 4 / S; sub++) {                                \
                 const int q = (4 / S) * (k0 / 4 + 8 * kk + 4 * h + i) + sub;                         \
-                if (q < P) {                                                                         \
-                    const double pinv = RESIDENT ? s_pinv[q] : pc[q].pinv;                           \
-                    _Pragma("unroll") for (int ct = 0; ct < 2; ct++) {                               \
-                        const uint32_t wd = (uint32_t)ycur[ct][kk][i];                               \
-                        const int32_t yc = S == 4 ? unslice4(wd) : unslice2(wd >> (16 * sub));       \
-                        gsum[ct] = fma((double)yc, pinv, gsum[ct]);                                  \
-                    }                                                                                \
+                /* no branch: rows past the last prime add yc * 0 */                                 \
+                const double pinv = RESIDENT ? s_pinv[q] : (q < P ? pc[min(q, P - 1)].pinv : 0.0);   \
+                _Pragma("unroll") for (int ct = 0; ct < 2; ct++) {                                   \
+                    const uint32_t wd = (uint32_t)ycur[ct][kk][i];                                   \
+                    const int32_t yc = S == 4 ? unslice4(wd) : unslice2(wd >> (16 * sub));           \
+                    gsum[ct] = fma((double)yc, pinv, gsum[ct]);                                      \
                 }                                                                                    \
             }                                                                                        \
         }                                                                                            \
