@@ -40,7 +40,8 @@ struct ReconJob {
     uint64_t *C;
     uint32_t *Wv;  // carried 32-bit words, nw x ystride
     int32_t *Cb;   // row-block carries, nrb x ystride
-    double *G;     // gamma partial sums, nrb x ystride
+    int64_t *G;    // fixed-point gamma partial sums, nrb x ystride
+    int gshift;    // their fraction bits
 };
 struct PlainJob {
     int64_t K, M, N, Crs, Ars, Brs, W, cap;
@@ -139,8 +140,8 @@ __global__ __launch_bounds__(256) void batch_recon_carry_kernel(Queue q, const R
     int64_t t;
     while (next_item(q, j, t)) {
         const ReconJob &r = jobs[j];
-        recon_carry_block(t, r.Wv, r.Cb, r.ystride, r.nw, r.nrb, r.G, r.M32, r.nent, r.colsC, r.C, r.Crs, r.W, r.cap,
-                          err);
+        recon_carry_block(t, r.Wv, r.Cb, r.ystride, r.nw, r.nrb, r.G, r.gshift, r.M32, r.nent, r.colsC, r.C, r.Crs, r.W,
+                          r.cap, err);
         __syncthreads();
     }
 }
@@ -279,7 +280,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
         o_rb[i] = L.take(v.S * v.P * d.rpadB * d.kpad);
         o_y[i] = L.take(ceil_div(v.S * v.P, 4) * d.ystride * 4);
         o_wv[i] = L.take(v.nw * d.ystride * 4);
-        o_ga[i] = L.take(ceil_div(4 * v.nw, RECON_ROWS) * d.ystride * 12);   // G (binary64), then Cb (int32)
+        o_ga[i] = L.take(ceil_div(4 * v.nw, RECON_ROWS) * d.ystride * 12);   // G (int64), then Cb (int32)
     }
     const int64_t o_ctr = L.take(64 * 8);
     int64_t o_rj[2], o_rp[2], o_gj[4], o_gp[4], o_cj[2], o_cp[2], o_pj[NCLASS], o_pp[NCLASS];
@@ -329,7 +330,8 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             c.pc = v.pc; c.Mc = v.mc; c.M32 = v.m32; c.Y = g.Y; c.P = (int)v.P; c.Kr = (int)v.Kr;
             c.nrb = (int)ceil_div(4 * v.nw, RECON_ROWS); c.nw = (int)v.nw; c.ystride = d.ystride;
             c.nent = p.K * p.N; c.colsC = p.N; c.Crs = p.Crs; c.W = p.cap + 1; c.cap = p.cap; c.C = (uint64_t *)p.C;
-            c.Wv = (uint32_t *)(base + o_wv[i]); c.G = (double *)(base + o_ga[i]);
+            c.Wv = (uint32_t *)(base + o_wv[i]); c.G = (int64_t *)(base + o_ga[i]);
+            c.gshift = v.S == 2 ? 45 : 32;
             c.Cb = (int32_t *)(base + o_ga[i] + c.nrb * d.ystride * 8);
             cj[sc].push_back(c);
             cp[sc].push_back(cp[sc].back() + ceil_div(c.nent, RECON_ENT) * c.nrb);

@@ -143,7 +143,7 @@ template <int S, bool RESIDENT>
 __global__ __launch_bounds__(256, 2) void crt_recon_gemm_kernel(
     const PrimeConst *__restrict__ pc, int P, const int8_t *__restrict__ Mc, int Kr, int nrb, int nw,
     const uint32_t *__restrict__ Y, int64_t ystride, int64_t nent, uint32_t *__restrict__ Wv,
-    int32_t *__restrict__ Cb, double *__restrict__ G, int64_t total)
+    int32_t *__restrict__ Cb, int64_t *__restrict__ G, int64_t total)
 {
     // XCD-aware order: the row blocks of one entry block share an L2
     int64_t per = (total + 7) / 8;
@@ -154,11 +154,11 @@ __global__ __launch_bounds__(256, 2) void crt_recon_gemm_kernel(
 
 __global__ __launch_bounds__(256) void crt_recon_carry_kernel(
     const uint32_t *__restrict__ Wv, const int32_t *__restrict__ Cb, int64_t ystride, int nw, int nrb,
-    const double *__restrict__ G,
+    const int64_t *__restrict__ G, int gshift,
     const uint32_t *__restrict__ M32, int64_t nent, int64_t colsC, uint64_t *__restrict__ C, int64_t Crs, int64_t W,
     int64_t cap, unsigned *__restrict__ err)
 {
-    recon_carry_block(blockIdx.x, Wv, Cb, ystride, nw, nrb, G, M32, nent, colsC, C, Crs, W, cap, err);
+    recon_carry_block(blockIdx.x, Wv, Cb, ystride, nw, nrb, G, gshift, M32, nent, colsC, C, Crs, W, cap, err);
 }
 
 }  // namespace mgx
@@ -352,13 +352,9 @@ int mgx_crt_plan_create(mgx_crt_plan **out, int64_t K, int64_t bitsA, int64_t bi
         for (int d = 0; d < 7; d++) {
             uint64_t w = powmod64(2, 8 * d, p);
             c.w[d] = w > p / 2 ? (int32_t)w - (int32_t)p : (int32_t)w;
-            if (d < 3) {
-                uint64_t wv = w * c.inv % p;
-                c.wi[d] = wv > p / 2 ? (int32_t)wv - (int32_t)p : (int32_t)wv;
-            }
         }
-        c.pinv = 1.0 / (double)p;
         c.off32 = p < 65536 ? (uint32_t)(((1ull << 31) / p + 1) * p) : 0;
+        c.f45 = p < 65536 ? (int32_t)((1ull << 45) / p) : 0;
         uint64_t ks = 0, pw = 1;
         for (int64_t b = 0; b < pl->kb; b++) {
             // reduce-GEMM constants: digits of centred 2^(8b) mod p
@@ -494,7 +490,7 @@ int recon_row_blocks(const mgx_crt_plan *pl) { return (int)ceil_div(4 * pl->nw, 
 
 // Both reconstruct passes with caller-provided scratch: Wv holds nw x ystride
 // 32-bit words, then Cb, the row blocks' carries (nrb x ystride int32), and
-// G, their partial gamma sums (nrb x ystride binary64).
+// G, their fixed-point partial gamma sums (nrb x ystride int64).
 int reconstruct_with(const mgx_crt_plan *pl, int64_t K, int64_t N, const uint32_t *Y, mgx_limb *C, int64_t Crs,
                      int64_t cap, char *scr, hipStream_t s)
 {
@@ -502,7 +498,7 @@ int reconstruct_with(const mgx_crt_plan *pl, int64_t K, int64_t N, const uint32_
     const int nrb = recon_row_blocks(pl);
     uint32_t *Wv = (uint32_t *)scr;
     int32_t *Cb = (int32_t *)(scr + pl->nw * ystride * 4);
-    double *G = (double *)(scr + (pl->nw + nrb) * ystride * 4);
+    int64_t *G = (int64_t *)(scr + (pl->nw + nrb) * ystride * 4);
     const int64_t lds = recon_res_lds(pl->Kr, pl->S);
     const bool resident = lds <= RECON_RES_MAX_LDS;
     // resident workgroups sweep RECON_EPW entry blocks each
@@ -515,7 +511,8 @@ int reconstruct_with(const mgx_crt_plan *pl, int64_t K, int64_t N, const uint32_
                        (int)pl->P, pl->d_mc, (int)pl->Kr, nrb, (int)pl->nw, Y, ystride, nent, Wv, Cb, G, total);
     MGX_HIP(hipGetLastError());
     hipLaunchKernelGGL(crt_recon_carry_kernel, dim3((unsigned)ceil_div(nent, CARRY_ENT)), dim3(256), 0, s, Wv, Cb,
-                       ystride, (int)pl->nw, nrb, G, pl->d_m32, nent, N, (uint64_t *)C, Crs, cap + 1, cap,
+                       ystride, (int)pl->nw, nrb, G, pl->S == 2 ? 45 : 32, pl->d_m32, nent, N, (uint64_t *)C, Crs,
+                       cap + 1, cap,
                        device_error_word());
     MGX_HIP(hipGetLastError());
     return MGX_OK;

@@ -25,9 +25,8 @@ struct PrimeConst {
     uint64_t fx;     // floor(2^63 / p)
     int32_t w[7];    // 2^(8d) mod p, centred
     uint32_t kofs;   // 128 * sum_{b<kb} 2^(8b) mod p
-    int32_t wi[3];   // 2^(8d) (M/p)^-1 mod p, centred
     uint32_t off32;  // a multiple of p in (2^31, 2^31 + p] (p < 2^16)
-    double pinv;     // 1/p, correctly rounded
+    int32_t f45;     // floor(2^45 / p) (p < 2^16)
 };
 
 __device__ __forceinline__ uint32_t mod_u64(uint64_t u, uint32_t p, uint64_t mu)
@@ -40,11 +39,12 @@ __device__ __forceinline__ uint32_t mod_s64(int64_t x, const PrimeConst &c)
 {
     return mod_u64((uint64_t)(x + (int64_t)c.off), c.p, c.mu);   // |x| < 2^62
 }
-// u mod p for p < 2^16 and any u < 2^32: with mu32 = floor(2^32 / p)
-// (= mu >> 32) the Barrett quotient is short by at most 2, so r < 3p
+// u mod p for 2^14 <= p < 2^16 and any u < 2^32: with mu32 = floor(2^32 / p)
+// (= mu >> 32) the Barrett quotient q is short by at most 2, so r < 3p; q
+// < 2^18, so q p takes the 24-bit multiplier (only its low 32 bits matter)
 __device__ __forceinline__ uint32_t bred32(uint32_t u, uint32_t p, uint32_t mu32)
 {
-    uint32_t r = u - __umulhi(u, mu32) * p;
+    uint32_t r = u - __umul24(__umulhi(u, mu32), p);
     r = min(r, r - p);   // r - p wraps above r when r < p
     return min(r, r - p);
 }
@@ -223,7 +223,7 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
 {
     __shared__ __attribute__((aligned(16))) int8_t LC[64 * RES_PITCH];
     __shared__ __attribute__((aligned(16))) int8_t LO[64 * 128];
-    __shared__ __attribute__((aligned(16))) uint32_t s_pc[32][4];   // S = 2: 1/p (two words), p, kofs
+    __shared__ __attribute__((aligned(16))) uint32_t s_pc[32][4];   // S = 2: p, kofs, floor(2^32/p), off32
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t e0 = blk * 128;
     const int cpe = kb / 16;   // 16-byte chunks per entry
@@ -277,8 +277,7 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
 #define PCLOAD(rb_)                                                                                   \
     if (S == 2 && tid < 32) {                                                                         \
         const PrimeConst &c_ = pc[min((rb_) * 32 + tid, P - 1)];                                      \
-        const uint64_t pi_ = __double_as_longlong(c_.pinv);                                           \
-        pcv[0] = (uint32_t)pi_; pcv[1] = (uint32_t)(pi_ >> 32); pcv[2] = c_.p; pcv[3] = c_.kofs;        \
+        pcv[0] = c_.p; pcv[1] = c_.kofs; pcv[2] = (uint32_t)(c_.mu >> 32); pcv[3] = c_.off32;          \
     }
     CLOAD(0);
     PCLOAD(0);
@@ -319,17 +318,14 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
                     const int ql = (rt * 32 + 8 * g + 4 * (lane >> 5)) / S + w, q = rb * (64 / S) + ql;
                     uint32_t word = 0;
                     if (S == 2) {
-                        // |acc| <= 2^22 (kb <= 256): v = acc0 + 256 acc1 + kofs is an int32
-                        // and |v / p| < 2^17, so in binary64 rint(v / p) is the nearest
-                        // quotient (v / p is at least 1/(2p) from a half-integer) and
-                        // v - rint(v / p) p the centred residue; the residue of -x is -r.
-                        // Computed for every lane and selected afterwards, so that the
-                        // constants' LDS reads are not each behind a branch.
-                        const uint4 k4 = *(const uint4 *)s_pc[ql];   // 1/p, p, kofs
-                        const int32_t v = acc[rt][4 * g + S * w] + acc[rt][4 * g + S * w + 1] * 256 + (int32_t)k4.w;
-                        const double y = (double)(neg ? -v : v);
-                        const double pinv = __longlong_as_double(((int64_t)k4.y << 32) | k4.x);
-                        const uint32_t r = slice2((int32_t)fma(-rint(y * pinv), (double)k4.z, y));
+                        // |acc| <= 2^22 (kb <= 256), so v + kofs + off32 lies in (0, 2^32):
+                        // one 32-bit Barrett step; the residue of -x is -r.  Computed for
+                        // every lane and selected afterwards, so that the constants' LDS
+                        // reads are not each behind a branch.
+                        const uint4 k4 = *(const uint4 *)s_pc[ql];   // p, kofs, mu32, off32
+                        const int32_t v = acc[rt][4 * g + S * w] + acc[rt][4 * g + S * w + 1] * 256;
+                        const int32_t c = centre(bred32((uint32_t)v + k4.y + k4.w, k4.x, k4.z), k4.x);
+                        const uint32_t r = slice2(neg ? -c : c);
                         word = (real && q < P) ? r : 0u;
                     } else if (S == 4 && real && q < P) {
                         const PrimeConst c = pc[q];
@@ -587,9 +583,7 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
         const int q = 2 * pair + half;
         if (q >= P) break;
         const PrimeConst c = pc[q];
-        const double pd = (double)c.p, pinv = c.pinv;
-        const double mw1 = (double)c.w[1], mw2 = (double)c.w[2];
-        const double fw0 = (double)c.wi[0], fw1 = (double)c.wi[1], fw2 = (double)c.wi[2];
+        const uint32_t mu32 = (uint32_t)(c.mu >> 32), off32 = c.off32;
         const int8_t *gA = RA + (int64_t)q * 2 * planeA;
         const int8_t *gB = RB + (int64_t)q * 2 * planeB;
         // this wave's DMA sources per stage; DMA instruction idx fills 16 rows
@@ -629,14 +623,11 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
 #pragma unroll
                 for (int ct = 0; ct < 2; ct++) acc[d][rt][ct] = (v16i){0};
 
-        // y = (acc0 + prev) w0 + acc1 w1 + acc2 w2 with w_d = 2^(8d) mod p,
-        // times (M/p)^-1 at the last fold, centred (|w_d| < 2^15); prev is the
-        // centred residue of earlier chunks.  |acc_d| <= 2^30 within a chunk,
-        // so every partial sum is an integer below 2^47 and exact in double.
-        // y/p is at least 1/(2p) > 2^-17 away from a half-integer (p odd) and
-        // |y/p| <= 2^32 makes y * pinv off by less than 2^-19, so rint gives
-        // the nearest quotient and y - rint(y pinv) p is the centred residue,
-        // exactly.
+        // x = acc0 + 256 (acc1 + 256 acc2) mod p plus the running residue of
+        // earlier chunks (kept in [0, p) in Y), in signed 32-bit Barrett steps
+        // (|acc_d| <= 2^30 within a chunk, every intermediate below 2^31 - 2^17);
+        // the last fold turns it into y = x (M/p)^-1 mod p, centred, as two
+        // balanced digits
 #define FOLD2(first_, last_, h_)                                                                     \
         _Pragma("unroll") for (int rt = 0; rt < 2; rt++)                                             \
         _Pragma("unroll") for (int ct = 0; ct < 2; ct++)                                             \
@@ -645,12 +636,11 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
             const int64_t j = j0 + wj * 64 + ct * 32 + l32;                                          \
             uint16_t *yh = (uint16_t *)(Yw + i * colsC + j) + (h_);                                  \
             const bool in = i < rowsC && j < colsC;                                                  \
-            const int32_t prev = (first_ || !in) ? 0 : (int32_t)(int16_t)*yh;                        \
-            const double y = fma((double)acc[2][rt][ct][r], (last_) ? fw2 : mw2,                     \
-                                 fma((double)acc[1][rt][ct][r], (last_) ? fw1 : mw1,                 \
-                                     (double)(acc[0][rt][ct][r] + prev) * ((last_) ? fw0 : 1.0)));   \
-            const int32_t cr = (int32_t)fma(-rint(y * pinv), pd, y);                                 \
-            const uint32_t res = (last_) ? slice2(cr) : ((uint32_t)cr & 0xffffu);                    \
+            const int32_t prev = (first_ || !in) ? 0 : (int32_t)*yh;                                 \
+            const uint32_t r2 = smod32(acc[2][rt][ct][r], c.p, mu32, off32);                         \
+            const uint32_t z = smod32(acc[1][rt][ct][r] + 256 * (int32_t)r2, c.p, mu32, off32);     \
+            uint32_t res = smod32(acc[0][rt][ct][r] + 256 * (int32_t)z + prev, c.p, mu32, off32);   \
+            if (last_) res = slice2(centre(bred32(__umul24(res, c.inv), c.p, mu32), c.p));           \
             if (!CHUNKED && last_) {                                                                 \
                 if ((h_) == 0) {                                                                     \
                     if (r & 1) hold[rt][ct][r >> 1] |= res << 16;                                    \
@@ -760,11 +750,12 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
 //        block's carry-out (|c| < 2^18) in Cb[rb][e], which belongs at word
 //        32 (rb + 1), or at word nw for the top block.  The
 //        workgroups also form gamma = round(sum_q yc_q / p_q) = round(X'/M),
-//        so that C = X' - gamma M (|C| <= M/4): row block rb takes the K
-//        steps ks = rb mod nrb and writes its partial sum, a binary64, to
-//        G[rb][e].  Each term is below 1/2 in magnitude, so with at most
-//        2^15 primes the rounding error of the sum stays below 2^-24, far
-//        inside the 1/4 margin.
+//        so that C = X' - gamma M (|C| <= M/4), in fixed point: row block rb
+//        takes the K steps ks = rb mod nrb and writes its partial sum to
+//        G[rb][e] (int64; S = 2: sum of yc floor(2^45/p), each term short by
+//        less than 2^-30; S = 4: sum of floor(y floor(2^63/p) / 2^31) - 2^32
+//        [yc < 0] with y = yc mod p, each short by less than 2^-31), far
+//        inside the 1/4 margin for any prime count the plans allow.
 //    (b) recon_carry_block: one thread per entry finds the sign of
 //        X' - gamma M from its top words down (the top block's carry first),
 //        then runs the carry chain of
@@ -788,19 +779,19 @@ __device__ __forceinline__ int64_t recon_idx(int64_t t, int64_t e, int64_t nt)
     return (e / RECON_ENT) * nt * RECON_ENT + t * RECON_ENT + e % RECON_ENT;
 }
 static_assert(RECON_ROWS / 4 == 32, "the carry pass takes one row block of words per 16-limb chunk");
-// (1/p for every digit row q < Kr/S, zero past the last prime)
-inline int64_t recon_res_lds(int64_t Kr, int64_t S) { return (int64_t)RECON_ROWS * (Kr + 16) + 8 * (Kr / S); }
+// (S = 2: floor(2^45/p) for every digit row q < Kr/S, zero past the last prime)
+inline int64_t recon_res_lds(int64_t Kr, int64_t S) { return (int64_t)RECON_ROWS * (Kr + 16) + 4 * (Kr / S); }
 
 template <int S, bool RESIDENT>
 __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *__restrict__ pc, int P,
     const int8_t *__restrict__ Mc, int Kr, int nrb, int nw, const uint32_t *__restrict__ Y, int64_t ystride,
-    int64_t nent, uint32_t *__restrict__ Wv, int32_t *__restrict__ Cb, double *__restrict__ G)
+    int64_t nent, uint32_t *__restrict__ Wv, int32_t *__restrict__ Cb, int64_t *__restrict__ G)
 {
     __shared__ __attribute__((aligned(16))) int8_t LAs[RESIDENT ? 16 : RECON_ROWS * LDS_PITCH];
     extern __shared__ __attribute__((aligned(16))) int8_t LDYN[];
     int8_t *const LA = RESIDENT ? LDYN : LAs;
     const int apitch = RESIDENT ? Kr + 16 : LDS_PITCH;
-    double *const s_pinv = (double *)(LDYN + RECON_ROWS * (Kr + 16));   // RESIDENT only
+    int32_t *const s_f45 = (int32_t *)(LDYN + RECON_ROWS * (Kr + 16));   // RESIDENT only
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, h = lane >> 5, c32 = lane & 31;
     const int rb = (int)(blk % nrb);
     const int64_t eb0 = (blk / nrb) * (RESIDENT ? RECON_EPW : 1);
@@ -821,7 +812,8 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
                 if (c < nch) *(v4i *)(LA + row * apitch + 16 * part) = t[u];
             }
         }
-        for (int q = tid; q < Kr / S; q += 256) s_pinv[q] = q < P ? pc[q].pinv : 0.0;
+        if (S == 2)
+            for (int q = tid; q < Kr / S; q += 256) s_f45[q] = q < P ? pc[q].f45 : 0;
         __syncthreads();
     }
 
@@ -839,7 +831,7 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
     // register sets that rotate by unrolling the loop by three, so that no
     // copy waits on a load just issued.
     v4i pa[2], y0[2][2], y1[2][2], y2[2][2];
-    double gsum[2] = {0.0, 0.0};
+    int64_t gsum[2] = {0, 0};
     // A: 128 rows x 64 bytes of Mc, two 16-byte chunks per thread
 #define ALOAD(dst, k0_)                                                                                \
     _Pragma("unroll") for (int u = 0; u < 2; u++) {                                                    \
@@ -874,12 +866,22 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
             _Pragma("unroll") for (int i = 0; i < 4; i++)                                            \
             _Pragma("unroll") for (int sub = 0; sub < 4 / S; sub++) {                                \
                 const int q = (4 / S) * (k0 / 4 + 8 * kk + 4 * h + i) + sub;                         \
-                /* no branch: rows past the last prime add yc * 0 */                                 \
-                const double pinv = RESIDENT ? s_pinv[q] : (q < P ? pc[min(q, P - 1)].pinv : 0.0);   \
-                _Pragma("unroll") for (int ct = 0; ct < 2; ct++) {                                   \
-                    const uint32_t wd = (uint32_t)ycur[ct][kk][i];                                   \
-                    const int32_t yc = S == 4 ? unslice4(wd) : unslice2(wd >> (16 * sub));           \
-                    gsum[ct] = fma((double)yc, pinv, gsum[ct]);                                      \
+                /* no branch: rows past the last prime add nothing */                                \
+                if (S == 2) {                                                                        \
+                    const int32_t f = RESIDENT ? s_f45[q] : (q < P ? pc[min(q, P - 1)].f45 : 0);     \
+                    _Pragma("unroll") for (int ct = 0; ct < 2; ct++) {                               \
+                        const int32_t yc = unslice2((uint32_t)ycur[ct][kk][i] >> (16 * sub));         \
+                        gsum[ct] += (int64_t)yc * f;                                                 \
+                    }                                                                                \
+                } else {                                                                             \
+                    const PrimeConst &c_ = pc[min(q, P - 1)];                                        \
+                    const uint32_t p_ = c_.p;                                                        \
+                    const uint64_t fx_ = q < P ? c_.fx : 0;                                          \
+                    _Pragma("unroll") for (int ct = 0; ct < 2; ct++) {                               \
+                        const int32_t yc = unslice4((uint32_t)ycur[ct][kk][i]);                      \
+                        const uint32_t y = yc < 0 ? (uint32_t)(yc + (int32_t)p_) : (uint32_t)yc;     \
+                        gsum[ct] += (int64_t)(((uint64_t)y * fx_) >> 31) - (yc < 0 && q < P ? (1ll << 32) : 0); \
+                    }                                                                                \
                 }                                                                                    \
             }                                                                                        \
         }                                                                                            \
@@ -941,9 +943,9 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
     }
 #pragma unroll
     for (int ct = 0; ct < 2; ct++) {   // lanes l and l + 32 hold the two halves of an entry's sum
-        const uint64_t b = (uint64_t)__double_as_longlong(gsum[ct]);
+        const uint64_t b = (uint64_t)gsum[ct];
         const uint32_t lo = __shfl_xor((unsigned)b, 32), hi = __shfl_xor((unsigned)(b >> 32), 32);
-        const double f = gsum[ct] + __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+        const int64_t f = gsum[ct] + (int64_t)(((uint64_t)hi << 32) | lo);
         const int64_t e = e0 + 64 * wave + 32 * ct + c32;
         if (h == 0 && e < nent) G[recon_idx(rb, e, nrb)] = f;
     }
@@ -954,7 +956,7 @@ constexpr int CARRY_ENT = 256;   // entries per workgroup
 constexpr int CARRY_PITCH = 17;  // limbs per LDS row (16 + 1 padding)
 
 __device__ __forceinline__ void recon_carry_block(int64_t blk, const uint32_t *__restrict__ Wv,
-    const int32_t *__restrict__ Cb, int64_t ystride, int nw, int nrb, const double *__restrict__ G,
+    const int32_t *__restrict__ Cb, int64_t ystride, int nw, int nrb, const int64_t *__restrict__ G, int gshift,
     const uint32_t *__restrict__ M32, int64_t nent, int64_t colsC,
     uint64_t *__restrict__ C, int64_t Crs, int64_t W, int64_t cap, unsigned *__restrict__ err)
 {
@@ -965,10 +967,10 @@ __device__ __forceinline__ void recon_carry_block(int64_t blk, const uint32_t *_
     const bool valid = e < nent;
     s_base[tid] = valid ? ((e / colsC) * Crs + e % colsC) * W : -1;
     int64_t gam = 0;
-    if (valid) {   // gamma from the row blocks' partial sums
-        double f = 0.0;
+    if (valid) {   // gamma: the rounded sum of the row blocks' fixed-point partial sums
+        int64_t f = 0;
         for (int rb = 0; rb < nrb; rb++) f += G[recon_idx(rb, e, nrb)];
-        gam = (int64_t)rint(f);
+        gam = (f + (1ll << (gshift - 1))) >> gshift;
     }
     // this entry's words and block carries: stride 256 within its entry block
     const uint32_t *wp = Wv + (valid ? recon_idx(0, e, nw) : 0);
