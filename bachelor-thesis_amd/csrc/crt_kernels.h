@@ -53,6 +53,12 @@ __device__ __forceinline__ uint32_t smod32(int32_t a, uint32_t p, uint32_t mu32,
 {
     return bred32((uint32_t)a + off32, p, mu32);
 }
+// the same without the corrections: a value congruent to a mod p in [0, 3p)
+__device__ __forceinline__ uint32_t smod32_lazy(int32_t a, uint32_t p, uint32_t mu32, uint32_t off32)
+{
+    const uint32_t u = (uint32_t)a + off32;
+    return u - __umul24(__umulhi(u, mu32), p);
+}
 // four balanced base-256 digits of |r| < 2^30, byte s = digit s
 __device__ __forceinline__ uint32_t slice4(int32_t r)
 {
@@ -625,7 +631,8 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
 
         // x = acc0 + 256 (acc1 + 256 acc2) mod p plus the running residue of
         // earlier chunks (kept in [0, p) in Y), in signed 32-bit Barrett steps
-        // (|acc_d| <= 2^30 within a chunk, every intermediate below 2^31 - 2^17);
+        // (|acc_d| <= 2^30 within a chunk; the two inner steps leave values
+        // below 3p < 2^18 uncorrected, so every sum stays below 2^31 - 2^17);
         // the last fold turns it into y = x (M/p)^-1 mod p, centred, as two
         // balanced digits
 #define FOLD2(first_, last_, h_)                                                                     \
@@ -637,8 +644,8 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
             uint16_t *yh = (uint16_t *)(Yw + i * colsC + j) + (h_);                                  \
             const bool in = i < rowsC && j < colsC;                                                  \
             const int32_t prev = (first_ || !in) ? 0 : (int32_t)*yh;                                 \
-            const uint32_t r2 = smod32(acc[2][rt][ct][r], c.p, mu32, off32);                         \
-            const uint32_t z = smod32(acc[1][rt][ct][r] + 256 * (int32_t)r2, c.p, mu32, off32);     \
+            const uint32_t r2 = smod32_lazy(acc[2][rt][ct][r], c.p, mu32, off32);                    \
+            const uint32_t z = smod32_lazy(acc[1][rt][ct][r] + 256 * (int32_t)r2, c.p, mu32, off32); \
             uint32_t res = smod32(acc[0][rt][ct][r] + 256 * (int32_t)z + prev, c.p, mu32, off32);   \
             if (last_) res = slice2(centre(bred32(__umul24(res, c.inv), c.p, mu32), c.p));           \
             if (!CHUNKED && last_) {                                                                 \
