@@ -771,8 +771,8 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
 // ------------------------------------------------------------------------
 constexpr int RECON_ROWS = 128;   // result bytes per workgroup
 constexpr int RECON_ENT = 256;    // entries per workgroup pass (64 per wave)
-// RESIDENT: the workgroup's whole row block of Mc (128 x Kr bytes) and 1/p of
-// every prime sit in dynamic LDS, loaded once, and the K loop runs without
+// RESIDENT: the workgroup's whole row block of Mc (128 x Kr bytes) and (S = 2)
+// floor(2^45/p) of every prime sit in dynamic LDS, loaded once, and the K loop runs without
 // barriers; the workgroup then sweeps RECON_EPW consecutive entry blocks with
 // them.  Used while that fits beside a second workgroup.
 constexpr int RECON_RES_MAX_LDS = 144 * 1024;
