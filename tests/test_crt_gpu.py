@@ -202,3 +202,62 @@ def test_crt_full_size_properties(gpu, slices):
     rows, cols = rng.sample(range(n), 3), rng.sample(range(n), 3)
     sub = oracle.mat_mul(np.ascontiguousarray(A[rows]), np.ascontiguousarray(B[:, cols]), cap)
     assert to_ints(sub) == [[entry_int(C[r, c]) for c in cols] for r in rows]
+
+
+def test_crt_stages_captured_in_graph(gpu, slices):
+    # The stage entry points only enqueue work; captured into a HIP graph
+    # (reconstruct then takes graph-owned scratch) and replayed, with eager
+    # work between the replays, every replay gives the eager product.  The
+    # graph calls go to the HIP runtime the library itself uses.
+    import ctypes
+    mgx = gpu
+    hip = ctypes.CDLL("libamdhip64.so")
+    vp = ctypes.c_void_p
+    K, M, N, bits, cap = 40, 33, 36, 400, 16
+    A, B = rand_ints(K, M, bits, 71), rand_ints(M, N, bits, 72)
+    plan = mgx.crt_plan(M, bits, bits)
+    stream = vp()
+    mgx.call("mgx_stream_create", ctypes.byref(stream))
+    graph, exe = vp(), vp()
+    try:
+        P = mgx.lib().mgx_crt_plan_primes(plan)
+        S = mgx.lib().mgx_crt_plan_slices(plan)
+        rp, kp, rpb = mgx.c_i64(), mgx.c_i64(), mgx.c_i64()
+        mgx.call("mgx_crt_residue_dims", plan, K, M, mgx.ctypes.byref(rp), mgx.ctypes.byref(kp))
+        mgx.call("mgx_crt_residue_dims", plan, N, M, mgx.ctypes.byref(rpb), None)
+        dA, dB = mgx.DeviceBuffer.from_array(to_entries(A, cap)), mgx.DeviceBuffer.from_array(to_entries(B, cap))
+        dRA = mgx.DeviceBuffer(S * P * rp.value * kp.value)
+        dRB = mgx.DeviceBuffer(S * P * rpb.value * kp.value)
+        dY = mgx.DeviceBuffer((S * P + 3) // 4 * mgx.lib().mgx_crt_y_stride(K, N) * 4)
+        dC = mgx.DeviceBuffer(K * N * (cap + 1) * 8)
+        want = matmul(A, B)
+
+        def stages(s):
+            mgx.call("mgx_crt_reduce", plan, 0, K, M, dA.ptr, M, cap, 0, dRA.ptr, s)
+            mgx.call("mgx_crt_reduce", plan, 1, N, M, dB.ptr, N, cap, 0, dRB.ptr, s)
+            mgx.call("mgx_crt_residue_gemm", plan, K, M, N, dRA.ptr, dRB.ptr, dY.ptr, s)
+            mgx.call("mgx_crt_reconstruct", plan, K, N, dY.ptr, dC.ptr, N, cap, s)
+
+        stages(None)
+        mgx.call("mgx_check_error", None)
+        assert to_ints(dC.to_array((K, N, cap + 1))) == want
+        assert hip.hipStreamBeginCapture(stream, 1) == 0   # thread-local capture
+        stages(stream.value)
+        assert hip.hipStreamEndCapture(stream, ctypes.byref(graph)) == 0
+        assert hip.hipGraphInstantiate(ctypes.byref(exe), graph, None, None, ctypes.c_size_t(0)) == 0
+        for rep in range(3):
+            dC.zero()
+            mgx.call("mgx_device_sync")
+            assert hip.hipGraphLaunch(exe, stream) == 0
+            mgx.call("mgx_stream_sync", stream.value)
+            mgx.call("mgx_check_error", None)
+            assert to_ints(dC.to_array((K, N, cap + 1))) == want, rep
+            stages(None)   # eager work between replays
+            mgx.call("mgx_device_sync")
+    finally:
+        if exe.value:
+            hip.hipGraphExecDestroy(exe)
+        if graph.value:
+            hip.hipGraphDestroy(graph)
+        mgx.call("mgx_stream_destroy", stream.value)
+        mgx.call("mgx_crt_plan_destroy", plan)
