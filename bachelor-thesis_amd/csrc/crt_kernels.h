@@ -789,6 +789,74 @@ static_assert(RECON_ROWS / 4 == 32, "the carry pass takes one row block of words
 // (S = 2: floor(2^45/p) for every digit row q < Kr/S, zero past the last prime)
 inline int64_t recon_res_lds(int64_t Kr, int64_t S) { return (int64_t)RECON_ROWS * (Kr + 16) + 4 * (Kr / S); }
 
+// Epilogue of a reconstruct tile (128 result bytes x 256 entries, lane
+// c32 + 32 h of wave w holds rows 8g + 4h + u of row tile rt for entries
+// e0 + 64 w + 32 ct + c32): the row block's 32 words of each entry, carried,
+// and its partial gamma sums.
+__device__ __forceinline__ void recon_store(const v16i (&acc)[4][2], const int64_t (&gsum)[2], int64_t e0, int wave,
+    int h, int c32, int rb, int nw, int nrb, int64_t nent, uint32_t *__restrict__ Wv, int32_t *__restrict__ Cb,
+    int64_t *__restrict__ G)
+{
+    // Row 8g + 4h + u of row tile rt is byte u of word j = rt*8 + 2g + h of
+    // the block (word rb*32 + j).  Lane c32 + 32h ends up with all 32 words
+    // of entry e0 + 64 wave + 32 h + c32: v_permlane32_swap exchanges the
+    // upper half-wave of the ct = 0 values with the lower half-wave of the
+    // ct = 1 values, after which the first operand holds word 2g' and the
+    // second word 2g' + 1 of the lane's own entry.
+    {
+        int64_t wv[32];
+#pragma unroll
+        for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                uint64_t v[2];
+#pragma unroll
+                for (int ct = 0; ct < 2; ct++)
+                    v[ct] = (uint64_t)((int64_t)acc[rt][ct][4 * g] + ((int64_t)acc[rt][ct][4 * g + 1] << 8) +
+                                       ((int64_t)acc[rt][ct][4 * g + 2] << 16) +
+                                       ((int64_t)acc[rt][ct][4 * g + 3] << 24));
+                const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)v[0], (unsigned)v[1], false, false);
+                const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(v[0] >> 32), (unsigned)(v[1] >> 32),
+                                                                 false, false);
+                wv[rt * 8 + 2 * g] = (int64_t)(((uint64_t)hi[0] << 32) | lo[0]);
+                wv[rt * 8 + 2 * g + 1] = (int64_t)(((uint64_t)hi[1] << 32) | lo[1]);
+            }
+        const int64_t e = e0 + 64 * wave + 32 * h + c32;
+        const int nv = min(RECON_ROWS / 4, nw - rb * (RECON_ROWS / 4));   // words of this block below nw
+        int64_t carry = 0;
+#pragma unroll
+        for (int j = 0; j < RECON_ROWS / 4; j++) {
+            if (j < nv) {
+                const int64_t x = wv[j] + carry;   // |wv| < 2^49
+                carry = x >> 32;
+                if (e < nent) Wv[recon_idx(rb * (RECON_ROWS / 4) + j, e, nw)] = (uint32_t)x;
+            }
+        }
+        if (e < nent) Cb[recon_idx(rb, e, nrb)] = (int32_t)carry;
+    }
+#pragma unroll
+    for (int ct = 0; ct < 2; ct++) {   // lanes l and l + 32 hold the two halves of an entry's sum
+        const uint64_t b = (uint64_t)gsum[ct];
+        const uint32_t lo = __shfl_xor((unsigned)b, 32), hi = __shfl_xor((unsigned)(b >> 32), 32);
+        const int64_t f = gsum[ct] + (int64_t)(((uint64_t)hi << 32) | lo);
+        const int64_t e = e0 + 64 * wave + 32 * ct + c32;
+        if (h == 0 && e < nent) G[recon_idx(rb, e, nrb)] = f;
+    }
+}
+
+// gamma term of digit row q (S = 2: word wd's half sub; S = 4: the whole word)
+// in the fixed point of the partial sums; zero for rows past the last prime
+template <int S>
+__device__ __forceinline__ int64_t gamma_term(uint32_t wd, int sub, int q, int P, const PrimeConst *__restrict__ pc,
+                                              int32_t f45)
+{
+    if (S == 2) return (int64_t)unslice2(wd >> (16 * sub)) * f45;
+    const PrimeConst &c_ = pc[min(q, P - 1)];
+    const int32_t yc = unslice4(wd);
+    const uint32_t y = yc < 0 ? (uint32_t)(yc + (int32_t)c_.p) : (uint32_t)yc;
+    return q < P ? (int64_t)(((uint64_t)y * c_.fx) >> 31) - (yc < 0 ? (1ll << 32) : 0) : 0;
+}
+
 template <int S, bool RESIDENT>
 __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *__restrict__ pc, int P,
     const int8_t *__restrict__ Mc, int Kr, int nrb, int nw, const uint32_t *__restrict__ Y, int64_t ystride,
@@ -874,22 +942,9 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
             _Pragma("unroll") for (int sub = 0; sub < 4 / S; sub++) {                                \
                 const int q = (4 / S) * (k0 / 4 + 8 * kk + 4 * h + i) + sub;                         \
                 /* no branch: rows past the last prime add nothing */                                \
-                if (S == 2) {                                                                        \
-                    const int32_t f = RESIDENT ? s_f45[q] : (q < P ? pc[min(q, P - 1)].f45 : 0);     \
-                    _Pragma("unroll") for (int ct = 0; ct < 2; ct++) {                               \
-                        const int32_t yc = unslice2((uint32_t)ycur[ct][kk][i] >> (16 * sub));         \
-                        gsum[ct] += (int64_t)yc * f;                                                 \
-                    }                                                                                \
-                } else {                                                                             \
-                    const PrimeConst &c_ = pc[min(q, P - 1)];                                        \
-                    const uint32_t p_ = c_.p;                                                        \
-                    const uint64_t fx_ = q < P ? c_.fx : 0;                                          \
-                    _Pragma("unroll") for (int ct = 0; ct < 2; ct++) {                               \
-                        const int32_t yc = unslice4((uint32_t)ycur[ct][kk][i]);                      \
-                        const uint32_t y = yc < 0 ? (uint32_t)(yc + (int32_t)p_) : (uint32_t)yc;     \
-                        gsum[ct] += (int64_t)(((uint64_t)y * fx_) >> 31) - (yc < 0 && q < P ? (1ll << 32) : 0); \
-                    }                                                                                \
-                }                                                                                    \
+                const int32_t f = S != 2 ? 0 : RESIDENT ? s_f45[q] : (q < P ? pc[min(q, P - 1)].f45 : 0); \
+                _Pragma("unroll") for (int ct = 0; ct < 2; ct++)                                     \
+                    gsum[ct] += gamma_term<S>((uint32_t)ycur[ct][kk][i], sub, q, P, pc, f);          \
             }                                                                                        \
         }                                                                                            \
         _Pragma("unroll") for (int kk = 0; kk < 2; kk++)                                             \
@@ -911,51 +966,7 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
 #undef RSTEP
 #undef ALOAD
 #undef YLOAD
-    // Row 8g + 4h + u of row tile rt is byte u of word j = rt*8 + 2g + h of
-    // the block (word rb*32 + j).  Lane c32 + 32h ends up with all 32 words
-    // of entry e0 + 64 wave + 32 h + c32: v_permlane32_swap exchanges the
-    // upper half-wave of the ct = 0 values with the lower half-wave of the
-    // ct = 1 values, after which the first operand holds word 2g' and the
-    // second word 2g' + 1 of the lane's own entry.
-    {
-        int64_t wv[32];
-#pragma unroll
-        for (int rt = 0; rt < 4; rt++)
-#pragma unroll
-            for (int g = 0; g < 4; g++) {
-                uint64_t v[2];
-#pragma unroll
-                for (int ct = 0; ct < 2; ct++)
-                    v[ct] = (uint64_t)((int64_t)acc[rt][ct][4 * g] + ((int64_t)acc[rt][ct][4 * g + 1] << 8) +
-                                       ((int64_t)acc[rt][ct][4 * g + 2] << 16) +
-                                       ((int64_t)acc[rt][ct][4 * g + 3] << 24));
-                const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)v[0], (unsigned)v[1], false, false);
-                const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(v[0] >> 32), (unsigned)(v[1] >> 32),
-                                                                 false, false);
-                wv[rt * 8 + 2 * g] = (int64_t)(((uint64_t)hi[0] << 32) | lo[0]);
-                wv[rt * 8 + 2 * g + 1] = (int64_t)(((uint64_t)hi[1] << 32) | lo[1]);
-            }
-        const int64_t e = e0 + 64 * wave + 32 * h + c32;
-        const int nv = min(RECON_ROWS / 4, nw - rb * (RECON_ROWS / 4));   // words of this block below nw
-        int64_t carry = 0;
-#pragma unroll
-        for (int j = 0; j < RECON_ROWS / 4; j++) {
-            if (j < nv) {
-                const int64_t x = wv[j] + carry;   // |wv| < 2^49
-                carry = x >> 32;
-                if (e < nent) Wv[recon_idx(rb * (RECON_ROWS / 4) + j, e, nw)] = (uint32_t)x;
-            }
-        }
-        if (e < nent) Cb[recon_idx(rb, e, nrb)] = (int32_t)carry;
-    }
-#pragma unroll
-    for (int ct = 0; ct < 2; ct++) {   // lanes l and l + 32 hold the two halves of an entry's sum
-        const uint64_t b = (uint64_t)gsum[ct];
-        const uint32_t lo = __shfl_xor((unsigned)b, 32), hi = __shfl_xor((unsigned)(b >> 32), 32);
-        const int64_t f = gsum[ct] + (int64_t)(((uint64_t)hi << 32) | lo);
-        const int64_t e = e0 + 64 * wave + 32 * ct + c32;
-        if (h == 0 && e < nent) G[recon_idx(rb, e, nrb)] = f;
-    }
+    recon_store(acc, gsum, e0, wave, h, c32, rb, nw, nrb, nent, Wv, Cb, G);
     }   // entry blocks
 }
 
