@@ -102,13 +102,19 @@ __global__ __launch_bounds__(256) void crt_reduce_kernel(
     reduce_tile<S>(blockIdx.x, pc, cst, P, nrb, kb, X, Xrs, W, transpose, rows, K, kpad, nent, R, plane, err);
 }
 
-template <int S>
+template <int S, int KS>
 __global__ __launch_bounds__(256) void crt_reduce_res_kernel(
     const PrimeConst *__restrict__ pc, const int8_t *__restrict__ cst, int P, int nrb, int kb,
     const uint64_t *__restrict__ X, int64_t Xrs, int64_t W, int transpose, int64_t rows, int64_t K,
     int64_t kpad, int8_t *__restrict__ R, int64_t plane, unsigned *__restrict__ err)
 {
-    reduce_tile_res<S>(blockIdx.x, pc, cst, P, nrb, kb, X, Xrs, W, transpose, rows, K, kpad, R, plane, err);
+    reduce_tile_res<S, KS>(blockIdx.x, pc, cst, P, nrb, kb, X, Xrs, W, transpose, rows, K, kpad, R, plane, err);
+}
+// the entry-resident reduce for a plan's byte bound kb (<= RES_KB, a multiple of 64)
+template <int S>
+static auto reduce_res_for(int64_t kb) -> decltype(&crt_reduce_res_kernel<S, 4>)
+{
+    return kb <= 64 ? crt_reduce_res_kernel<S, 1> : kb <= 128 ? crt_reduce_res_kernel<S, 2> : crt_reduce_res_kernel<S, 4>;
 }
 
 template <bool CHUNKED>
@@ -369,6 +375,7 @@ int mgx_crt_plan_create(mgx_crt_plan **out, int64_t K, int64_t bitsA, int64_t bi
             pw = (pw * 256) % p;
         }
         c.kofs = (uint32_t)(ks * 128 % p);
+        c.kneg = (uint32_t)((p - ks * 127 % p) % p);
         // reconstruct constants: digits of (M/p) 256^s
         for (int s = 0; s < S; s++) {
             hb::Nat v = hb::shl(Mq, 8 * s);
@@ -435,7 +442,7 @@ int mgx_crt_reduce(const mgx_crt_plan *pl, int transpose, int64_t rows, int64_t 
     const bool resident = pl->kb <= RES_KB;   // the entries' bytes fit LDS: read them once
     int64_t nblk = ceil_div(nent, 128) * (resident ? 1 : nrb);
     if (nblk > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "matrix too large for one reduce launch");
-    auto res = pl->S == 2 ? crt_reduce_res_kernel<2> : crt_reduce_res_kernel<4>;
+    auto res = pl->S == 2 ? reduce_res_for<2>(pl->kb) : reduce_res_for<4>(pl->kb);
     auto gen = pl->S == 2 ? crt_reduce_kernel<2> : crt_reduce_kernel<4>;
     if (resident)
         hipLaunchKernelGGL(res, dim3((unsigned)nblk), dim3(256), 0, as_stream(stream), pl->d_pc, pl->d_cst,

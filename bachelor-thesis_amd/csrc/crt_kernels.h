@@ -27,6 +27,7 @@ struct PrimeConst {
     uint32_t kofs;   // 128 * sum_{b<kb} 2^(8b) mod p
     uint32_t off32;  // a multiple of p in (2^31, 2^31 + p] (p < 2^16)
     int32_t f45;     // floor(2^45 / p) (p < 2^16)
+    uint32_t kneg;   // -127 * sum_{b<kb} 2^(8b) mod p
 };
 
 __device__ __forceinline__ uint32_t mod_u64(uint64_t u, uint32_t p, uint64_t mu)
@@ -39,25 +40,29 @@ __device__ __forceinline__ uint32_t mod_s64(int64_t x, const PrimeConst &c)
 {
     return mod_u64((uint64_t)(x + (int64_t)c.off), c.p, c.mu);   // |x| < 2^62
 }
-// u mod p for 2^14 <= p < 2^16 and any u < 2^32: with mu32 = floor(2^32 / p)
-// (= mu >> 32) the Barrett quotient q is short by at most 2, so r < 3p; q
-// < 2^18, so q p takes the 24-bit multiplier (only its low 32 bits matter)
-__device__ __forceinline__ uint32_t bred32(uint32_t u, uint32_t p, uint32_t mu32)
+// u mod p for 2^14 <= p < 2^16 and any u < 2^32, given negp = -p and mu32 =
+// floor(2^32 / p): the quotient q = floor(u mu32 / 2^32) is short of
+// floor(u / p) by at most 1 (u mu32 / 2^32 > u / p - u / 2^32 > u / p - 1),
+// so u - q p < 2p; q < 2^18 and p < 2^16 take the signed 24-bit
+// multiply-add (only the low 32 bits of the result matter)
+__device__ __forceinline__ uint32_t bred32_lazy(uint32_t u, int32_t negp, uint32_t mu32)
 {
-    uint32_t r = u - __umul24(__umulhi(u, mu32), p);
-    r = min(r, r - p);   // r - p wraps above r when r < p
-    return min(r, r - p);
+    return u + (uint32_t)__mul24((int32_t)__umulhi(u, mu32), negp);
+}
+__device__ __forceinline__ uint32_t bred32(uint32_t u, int32_t negp, uint32_t mu32)
+{
+    const uint32_t r = bred32_lazy(u, negp, mu32);
+    return min(r, r + (uint32_t)negp);   // r - p wraps above r when r < p
 }
 // a mod p for signed |a| < 2^31 - 2^17: off32 is a multiple of p in (2^31, 2^31 + p]
-__device__ __forceinline__ uint32_t smod32(int32_t a, uint32_t p, uint32_t mu32, uint32_t off32)
+__device__ __forceinline__ uint32_t smod32(int32_t a, int32_t negp, uint32_t mu32, uint32_t off32)
 {
-    return bred32((uint32_t)a + off32, p, mu32);
+    return bred32((uint32_t)a + off32, negp, mu32);
 }
-// the same without the corrections: a value congruent to a mod p in [0, 3p)
-__device__ __forceinline__ uint32_t smod32_lazy(int32_t a, uint32_t p, uint32_t mu32, uint32_t off32)
+// the same without the correction: a value congruent to a mod p in [0, 2p)
+__device__ __forceinline__ uint32_t smod32_lazy(int32_t a, int32_t negp, uint32_t mu32, uint32_t off32)
 {
-    const uint32_t u = (uint32_t)a + off32;
-    return u - __umul24(__umulhi(u, mu32), p);
+    return bred32_lazy((uint32_t)a + off32, negp, mu32);
 }
 // four balanced base-256 digits of |r| < 2^30, byte s = digit s
 __device__ __forceinline__ uint32_t slice4(int32_t r)
@@ -217,19 +222,24 @@ __device__ __forceinline__ void reduce_tile(int64_t blk,
 // half of one entry's bytes in registers for the whole sweep (they are the
 // MFMA B operand of every prime block), and the workgroup sweeps every prime
 // block over its 128 entries; the residues of a prime block leave through
-// LDS as 8-byte row segments.
+// LDS as 8-byte row segments.  KS: 64-byte K steps (kb <= 64 KS; the
+// constant columns kb .. 64 KS are zero).
 constexpr int RES_KB = 256;
 constexpr int RES_PITCH = RES_KB + 16;
 
-template <int S>
+template <int S, int KS>
 __device__ __forceinline__ void reduce_tile_res(int64_t blk,
     const PrimeConst *__restrict__ pc, const int8_t *__restrict__ cst, int P, int nrb, int kb,
     const uint64_t *__restrict__ X, int64_t Xrs, int64_t W, int transpose, int64_t rows, int64_t K,
     int64_t kpad, int8_t *__restrict__ R, int64_t plane, unsigned *__restrict__ err)
 {
+    static_assert(KS >= 1 && 64 * KS <= RES_KB, "K steps");
+    constexpr int K32 = 2 * KS;   // 32-byte MFMA steps
     __shared__ __attribute__((aligned(16))) int8_t LC[64 * RES_PITCH];
     __shared__ __attribute__((aligned(16))) int8_t LO[64 * 128];
-    __shared__ __attribute__((aligned(16))) uint32_t s_pc[32][4];   // S = 2: p, kofs, floor(2^32/p), off32
+    // S = 2, per sign of the entry and prime of the row block: -p, floor(2^32/p),
+    // the offset K and 128 - floor(p/2) (see the epilogue)
+    __shared__ __attribute__((aligned(16))) uint32_t s_pc[2][32][4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t e0 = blk * 128;
     const int cpe = kb / 16;   // 16-byte chunks per entry
@@ -238,27 +248,29 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
     // This lane's entry, bytes 32 k32 + 16 hh .. + 15 for every k32 < kb/32
     // (limbs 4 k32 + 2 hh, + 1): loaded with no condition, all in flight at
     // once (padding reads entry 0, limb indices clamped below the cap), then
-    // masked at |size| and biased to byte - 128.
-    bool real, neg;
-    v4i bf[RES_KB / 32];
+    // masked at |size| and biased to byte - 128, or (S = 2, negative entry) to
+    // 127 - byte, which makes the products sum to -|x| plus a constant.
+    bool neg;
+    v4i bf[K32];
     {
         const int64_t e = e0 + el, r = e / kpad, k = e % kpad;
-        real = r < rows && k < K;
+        const bool real = r < rows && k < K;
         const int64_t base = real ? (transpose ? (k * Xrs + r) * W : (r * Xrs + k) * W) : 0;
         const int lim = (int)min<int64_t>(kb / 8, W - 1);
         const int64_t sz = (int64_t)X[base + W - 1];
-        uint64_t v[RES_KB / 32][2];
+        uint64_t v[K32][2];
 #pragma unroll
-        for (int k32 = 0; k32 < RES_KB / 32; k32++)
+        for (int k32 = 0; k32 < K32; k32++)
 #pragma unroll
             for (int u = 0; u < 2; u++) v[k32][u] = X[base + min(4 * k32 + 2 * hh + u, lim - 1)];
         const int64_t a = real ? min(sz < 0 ? -sz : sz, W - 1) : 0;
         const int nl = (int)min<int64_t>(a, lim);
         neg = real && sz < 0;
+        const uint64_t flip = (S == 2 && neg) ? 0x7f7f7f7f7f7f7f7full : 0x8080808080808080ull;
 #pragma unroll
-        for (int k32 = 0; k32 < RES_KB / 32; k32++) {
-            const uint64_t d0 = (4 * k32 + 2 * hh < nl ? v[k32][0] : 0) ^ 0x8080808080808080ull;
-            const uint64_t d1 = (4 * k32 + 2 * hh + 1 < nl ? v[k32][1] : 0) ^ 0x8080808080808080ull;
+        for (int k32 = 0; k32 < K32; k32++) {
+            const uint64_t d0 = (4 * k32 + 2 * hh < nl ? v[k32][0] : 0) ^ flip;
+            const uint64_t d1 = (4 * k32 + 2 * hh + 1 < nl ? v[k32][1] : 0) ^ flip;
             bf[k32] = (v4i){(int)(uint32_t)d0, (int)(uint32_t)(d0 >> 32), (int)(uint32_t)d1, (int)(uint32_t)(d1 >> 32)};
         }
         if (hh == 0)
@@ -275,15 +287,23 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
         cst_off[u] = c < 64 * cpe ? row * kb + 16 * part : -1;
         lc_off[u] = row * RES_PITCH + 16 * part;
     }
+    // columns kb .. 64 KS: zero once (the row blocks' copies never write them)
+    for (int c = tid; c < 64 * (4 * KS - cpe); c += 256) {
+        const int row = c / (4 * KS - cpe), part = cpe + c % (4 * KS - cpe);
+        *(v4i *)(LC + row * RES_PITCH + 16 * part) = (v4i){0, 0, 0, 0};
+    }
 #define CLOAD(rb_)                                                                                    \
     _Pragma("unroll") for (int u = 0; u < 4; u++)                                                     \
         if (cst_off[u] >= 0) cpre[u] = *(const v4i *)(cst + (int64_t)(rb_) * 64 * kb + cst_off[u]);
-    // S = 2: the row block's 32 primes' constants, one block ahead in registers
-    uint32_t pcv[4] = {0, 0, 0, 0};
+    // S = 2: the row block's 32 primes' epilogue constants (thread 32 sg + j:
+    // sign sg, prime j), one block ahead in registers
+    uint4 pcv = make_uint4(0, 0, 0, 0);
 #define PCLOAD(rb_)                                                                                   \
-    if (S == 2 && tid < 32) {                                                                         \
-        const PrimeConst &c_ = pc[min((rb_) * 32 + tid, P - 1)];                                      \
-        pcv[0] = c_.p; pcv[1] = c_.kofs; pcv[2] = (uint32_t)(c_.mu >> 32); pcv[3] = c_.off32;          \
+    if (S == 2 && tid < 64) {                                                                         \
+        const PrimeConst &c_ = pc[min((rb_) * 32 + (tid & 31), P - 1)];                               \
+        const uint32_t h_ = c_.p >> 1;                                                                \
+        pcv = make_uint4((uint32_t)-(int32_t)c_.p, (uint32_t)(c_.mu >> 32),                          \
+                         (tid >> 5 ? c_.kneg : c_.kofs) + c_.off32 + h_, 128u - h_);                  \
     }
     CLOAD(0);
     PCLOAD(0);
@@ -292,7 +312,7 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
 #pragma unroll
         for (int u = 0; u < 4; u++)
             if (cst_off[u] >= 0) *(v4i *)(LC + lc_off[u]) = cpre[u];
-        if (S == 2 && tid < 32) *(uint4 *)s_pc[tid] = make_uint4(pcv[0], pcv[1], pcv[2], pcv[3]);
+        if (S == 2 && tid < 64) *(uint4 *)s_pc[tid >> 5][tid & 31] = pcv;
         __syncthreads();
         if (rb + 1 < nrb) {
             CLOAD(rb + 1);
@@ -300,14 +320,12 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
         }
         v16i acc[2];
 #pragma unroll
-        for (int k32 = 0; k32 < RES_KB / 32; k32++) {
-            if (k32 * 32 < kb) {
-                const int ko = k32 * 32 + 16 * hh;
+        for (int k32 = 0; k32 < K32; k32++) {
+            const int ko = k32 * 32 + 16 * hh;
 #pragma unroll
-                for (int rt = 0; rt < 2; rt++) {
-                    const v4i af = *(const v4i *)(LC + (rt * 32 + (lane & 31)) * RES_PITCH + ko);
-                    acc[rt] = mfma32(af, bf[k32], k32 == 0 ? (v16i){0} : acc[rt]);
-                }
+            for (int rt = 0; rt < 2; rt++) {
+                const v4i af = *(const v4i *)(LC + (rt * 32 + (lane & 31)) * RES_PITCH + ko);
+                acc[rt] = mfma32(af, bf[k32], k32 == 0 ? (v16i){0} : acc[rt]);
             }
         }
         // the residue digits of rows 4 qg .. 4 qg + 3 of this row block
@@ -319,30 +337,37 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
 #pragma unroll
             for (int g = 0; g < 4; g++) {
                 uint32_t dw = 0;
+                if (S == 2) {
+                    // |acc| <= 2^22 (kb <= 256) and v = acc_0 + 256 acc_1 is
+                    // congruent to x - 128 sum 2^(8b) (to -x + 127 sum 2^(8b) for a
+                    // negative entry), so u = v + K with K = (128 resp. -127) sum
+                    // 2^(8b) mod p + off32 + h lies in [0, 2^32) and is congruent
+                    // to x + h; with h = floor(p/2), (u mod p) - h is the centred
+                    // residue and (u mod p) + 128 - h its two digits' bytes plus
+                    // 128.  Padding entries have no limbs and come out 0.
+                    const uint4 *tab = (const uint4 *)s_pc[neg] + 16 * rt + 4 * g + 2 * hh;
+                    uint32_t xw[2];
 #pragma unroll
-                for (int w = 0; w < 4 / S; w++) {
-                    const int ql = (rt * 32 + 8 * g + 4 * (lane >> 5)) / S + w, q = rb * (64 / S) + ql;
-                    uint32_t word = 0;
-                    if (S == 2) {
-                        // |acc| <= 2^22 (kb <= 256), so v + kofs + off32 lies in (0, 2^32):
-                        // one 32-bit Barrett step; the residue of -x is -r.  Computed for
-                        // every lane and selected afterwards, so that the constants' LDS
-                        // reads are not each behind a branch.
-                        const uint4 k4 = *(const uint4 *)s_pc[ql];   // p, kofs, mu32, off32
-                        const int32_t v = acc[rt][4 * g + S * w] + acc[rt][4 * g + S * w + 1] * 256;
-                        const int32_t c = centre(bred32((uint32_t)v + k4.y + k4.w, k4.x, k4.z), k4.x);
-                        const uint32_t r = slice2(neg ? -c : c);
-                        word = (real && q < P) ? r : 0u;
-                    } else if (S == 4 && real && q < P) {
-                        const PrimeConst c = pc[q];
+                    for (int w = 0; w < 2; w++) {
+                        const uint4 k4 = tab[w];   // -p, mu32, K, 128 - h
+                        const uint32_t u = (uint32_t)(acc[rt][4 * g + 2 * w] + acc[rt][4 * g + 2 * w + 1] * 256) + k4.z;
+                        xw[w] = bred32(u, (int32_t)k4.x, k4.y) + k4.w;
+                    }
+                    dw = ((xw[0] & 0xffffu) | (xw[1] << 16)) ^ 0x00800080u;
+                } else {
+#pragma unroll
+                    for (int w = 0; w < 4 / S; w++) {
+                        const int ql = (rt * 32 + 8 * g + 4 * (lane >> 5)) / S + w, q = rb * (64 / S) + ql;
+                        // (rows past the last prime take its constants and are not stored;
+                        // padding entries have no limbs and come out 0)
+                        const PrimeConst c = pc[min(q, P - 1)];
                         int64_t v = c.kofs;
 #pragma unroll
                         for (int s = 0; s < S; s++) v += (int64_t)acc[rt][4 * g + S * w + s] << (8 * s);
                         uint32_t res = mod_s64(v, c);
                         if (neg && res) res = c.p - res;
-                        word = slice_s<S>(centre(res, c.p));
+                        dw |= slice_s<S>(centre(res, c.p)) << (16 * w);
                     }
-                    dw |= word << (16 * w);
                 }
                 LO4[(rt * 8 + 2 * g + (lane >> 5)) * 128 + el] = dw;
             }
@@ -589,7 +614,8 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
         const int q = 2 * pair + half;
         if (q >= P) break;
         const PrimeConst c = pc[q];
-        const uint32_t mu32 = (uint32_t)(c.mu >> 32), off32 = c.off32;
+        const uint32_t mu32 = (uint32_t)(c.mu >> 32), off32 = c.off32, hp = c.p >> 1, c128h = 128u - hp;
+        const int32_t negp = -(int32_t)c.p;
         const int8_t *gA = RA + (int64_t)q * 2 * planeA;
         const int8_t *gB = RB + (int64_t)q * 2 * planeB;
         // this wave's DMA sources per stage; DMA instruction idx fills 16 rows
@@ -632,9 +658,10 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
         // x = acc0 + 256 (acc1 + 256 acc2) mod p plus the running residue of
         // earlier chunks (kept in [0, p) in Y), in signed 32-bit Barrett steps
         // (|acc_d| <= 2^30 within a chunk; the two inner steps leave values
-        // below 3p < 2^18 uncorrected, so every sum stays below 2^31 - 2^17);
+        // below 2p < 2^17 uncorrected, so every sum stays below 2^31 - 2^17);
         // the last fold turns it into y = x (M/p)^-1 mod p, centred, as two
-        // balanced digits
+        // balanced digits: with h = floor(p/2), (x (M/p)^-1 + h) mod p - h is
+        // the centred residue, and adding 128 gives the digits' bytes
 #define FOLD2(first_, last_, h_)                                                                     \
         _Pragma("unroll") for (int rt = 0; rt < 2; rt++)                                             \
         _Pragma("unroll") for (int ct = 0; ct < 2; ct++)                                             \
@@ -644,10 +671,11 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
             uint16_t *yh = (uint16_t *)(Yw + i * colsC + j) + (h_);                                  \
             const bool in = i < rowsC && j < colsC;                                                  \
             const int32_t prev = (first_ || !in) ? 0 : (int32_t)*yh;                                 \
-            const uint32_t r2 = smod32_lazy(acc[2][rt][ct][r], c.p, mu32, off32);                    \
-            const uint32_t z = smod32_lazy(acc[1][rt][ct][r] + 256 * (int32_t)r2, c.p, mu32, off32); \
-            uint32_t res = smod32(acc[0][rt][ct][r] + 256 * (int32_t)z + prev, c.p, mu32, off32);   \
-            if (last_) res = slice2(centre(bred32(__umul24(res, c.inv), c.p, mu32), c.p));           \
+            const uint32_t r2 = smod32_lazy(acc[2][rt][ct][r], negp, mu32, off32);                   \
+            const uint32_t z = smod32_lazy(acc[1][rt][ct][r] + 256 * (int32_t)r2, negp, mu32, off32); \
+            uint32_t res = smod32(acc[0][rt][ct][r] + 256 * (int32_t)z + prev, negp, mu32, off32);  \
+            if (last_)   /* res inv + h < p^2 + p < 2^32 */                                          \
+                res = ((bred32(__umul24(res, c.inv) + hp, negp, mu32) + c128h) ^ 0x80u) & 0xffffu;    \
             if (!CHUNKED && last_) {                                                                 \
                 if ((h_) == 0) {                                                                     \
                     if (r & 1) hold[rt][ct][r >> 1] |= res << 16;                                    \
