@@ -226,6 +226,7 @@ __device__ __forceinline__ void reduce_tile(int64_t blk,
 // constant columns kb .. 64 KS are zero).
 constexpr int RES_KB = 256;
 constexpr int RES_PITCH = RES_KB + 16;
+constexpr int RES_KG = 2;   // K steps whose A fragments are read ahead of their MFMAs
 
 template <int S, int KS>
 __device__ __forceinline__ void reduce_tile_res(int64_t blk,
@@ -318,15 +319,22 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
             CLOAD(rb + 1);
             PCLOAD(rb + 1);
         }
+        // A fragments in groups of RES_KG K steps, all read before the group's
+        // MFMAs, so that the MFMAs do not each wait on an LDS read
         v16i acc[2];
 #pragma unroll
-        for (int k32 = 0; k32 < K32; k32++) {
-            const int ko = k32 * 32 + 16 * hh;
+        for (int k0 = 0; k0 < K32; k0 += RES_KG) {
+            v4i af[RES_KG][2];
 #pragma unroll
-            for (int rt = 0; rt < 2; rt++) {
-                const v4i af = *(const v4i *)(LC + (rt * 32 + (lane & 31)) * RES_PITCH + ko);
-                acc[rt] = mfma32(af, bf[k32], k32 == 0 ? (v16i){0} : acc[rt]);
-            }
+            for (int k32 = k0; k32 < k0 + RES_KG && k32 < K32; k32++)
+#pragma unroll
+                for (int rt = 0; rt < 2; rt++)
+                    af[k32 - k0][rt] = *(const v4i *)(LC + (rt * 32 + (lane & 31)) * RES_PITCH + k32 * 32 + 16 * hh);
+#pragma unroll
+            for (int k32 = k0; k32 < k0 + RES_KG && k32 < K32; k32++)
+#pragma unroll
+                for (int rt = 0; rt < 2; rt++)
+                    acc[rt] = mfma32(af[k32 - k0][rt], bf[k32], k32 == 0 ? (v16i){0} : acc[rt]);
         }
         // the residue digits of rows 4 qg .. 4 qg + 3 of this row block
         // (qg = 8 rt + 2 g + lane / 32; S = 2: primes 2 qg, 2 qg + 1 with two
