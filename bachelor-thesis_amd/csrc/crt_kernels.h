@@ -227,6 +227,10 @@ __device__ __forceinline__ void reduce_tile(int64_t blk,
 constexpr int RES_KB = 256;
 constexpr int RES_PITCH = RES_KB + 16;
 constexpr int RES_KG = 2;   // K steps whose A fragments are read ahead of their MFMAs
+// word of entry el in an LO4 row: the 16-byte group of entries el/4 moves to
+// group (el/4) ^ 1 in the upper half of the row, so that the transpose's
+// 16-byte reads of groups 2c and 2c + 1 (c = 0..15) are free of bank conflicts
+__device__ __forceinline__ int lo4_swz(int el) { return el ^ (((el >> 6) & 1) << 2); }
 
 template <int S, int KS>
 __device__ __forceinline__ void reduce_tile_res(int64_t blk,
@@ -239,8 +243,10 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
     __shared__ __attribute__((aligned(16))) int8_t LC[64 * RES_PITCH];
     __shared__ __attribute__((aligned(16))) int8_t LO[64 * 128];
     // S = 2, per sign of the entry and prime of the row block: -p, floor(2^32/p),
-    // the offset K and 128 - floor(p/2) (see the epilogue)
-    __shared__ __attribute__((aligned(16))) uint32_t s_pc[2][32][4];
+    // the offset K and 128 - floor(p/2) (see the epilogue); the negative
+    // entries' table starts at row 33, so that the two tables' rows of one
+    // prime sit in different banks
+    __shared__ uint4 s_pc[65];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t e0 = blk * 128;
     const int cpe = kb / 16;   // 16-byte chunks per entry
@@ -313,7 +319,7 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
 #pragma unroll
         for (int u = 0; u < 4; u++)
             if (cst_off[u] >= 0) *(v4i *)(LC + lc_off[u]) = cpre[u];
-        if (S == 2 && tid < 64) *(uint4 *)s_pc[tid >> 5][tid & 31] = pcv;
+        if (S == 2 && tid < 64) s_pc[(tid >> 5) * 33 + (tid & 31)] = pcv;
         __syncthreads();
         if (rb + 1 < nrb) {
             CLOAD(rb + 1);
@@ -353,7 +359,7 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
                     // to x + h; with h = floor(p/2), (u mod p) - h is the centred
                     // residue and (u mod p) + 128 - h its two digits' bytes plus
                     // 128.  Padding entries have no limbs and come out 0.
-                    const uint4 *tab = (const uint4 *)s_pc[neg] + 16 * rt + 4 * g + 2 * hh;
+                    const uint4 *tab = s_pc + (neg ? 33 : 0) + 16 * rt + 4 * g + 2 * hh;
                     uint32_t xw[2];
 #pragma unroll
                     for (int w = 0; w < 2; w++) {
@@ -377,7 +383,7 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
                         dw |= slice_s<S>(centre(res, c.p)) << (16 * w);
                     }
                 }
-                LO4[(rt * 8 + 2 * g + (lane >> 5)) * 128 + el] = dw;
+                LO4[(rt * 8 + 2 * g + (lane >> 5)) * 128 + lo4_swz(el)] = dw;
             }
         __syncthreads();
         {
@@ -386,8 +392,8 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
             // 16 threads of one qg write 128 consecutive bytes per row
             const int qg = tid >> 4, ch = tid & 15;
             const int64_t eo = res_off(S, (e0 + 8 * ch) / kpad, (e0 + 8 * ch) % kpad, kpad);   // 8 columns, one row
-            const uint4 a = *(const uint4 *)(LO4 + qg * 128 + 8 * ch);
-            const uint4 b = *(const uint4 *)(LO4 + qg * 128 + 8 * ch + 4);
+            const uint4 a = *(const uint4 *)(LO4 + qg * 128 + lo4_swz(8 * ch));
+            const uint4 b = *(const uint4 *)(LO4 + qg * 128 + lo4_swz(8 * ch + 4));
             const uint32_t d[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
             for (int k = 0; k < 4; k++) {
@@ -539,8 +545,10 @@ __device__ __forceinline__ void gemm_tile(int64_t v,
 //
 //     Stage layout: [A | B][slice][row 0..127][4 chunks of 16 bytes], the
 //     chunk of K bytes 16c..16c+15 of row r at slot c ^ g(r) with
-//     g(r) = ((r >> 1) ^ (r >> 3)) & 3, so that 8 (and 16) consecutive rows
-//     read at one K chunk hit distinct banks with 32 or 64 banks.  One DMA
+//     g(r) = (r >> 2) & 3: rows 64 bytes apart share banks every fourth row
+//     (64 four-byte banks), and g gives rows r, r + 4, r + 8, r + 12 distinct
+//     slots, which made the fragment reads free of bank conflicts
+//     (SQ_LDS_BANK_CONFLICT 0; XOR patterns on row bits 0-1 cost 2-4x).  One DMA
 //     instruction of a wave writes 1 KB (16 rows) at base + 16*lane; lane l
 //     takes row 16*rb16 + l/4, slot l%4, i.e. chunk (l & 3) ^ g(row).
 // ------------------------------------------------------------------------
@@ -557,7 +565,7 @@ template <int WM> struct G2Shape {
 typedef __attribute__((address_space(3))) void lds_void;
 
 __device__ __forceinline__ uint32_t lds_offset(const void *p) { return (uint32_t)(uintptr_t)(const lds_void *)p; }
-__device__ __forceinline__ int g2_swz(int r) { return ((r >> 1) ^ (r >> 3)) & 3; }
+__device__ __forceinline__ int g2_swz(int r) { return (r >> 2) & 3; }
 
 // 16 bytes from sbase + voff (sbase wave-uniform, voff per lane) to LDS at
 // lds_base + 16 * lane (lds_base wave-uniform).  Issued as inline assembly so
@@ -631,14 +639,13 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
         // else B slice (idx - A_DMA) / 8, rows 16 ((idx - A_DMA) % 8).  The
         // 16-row block's start is a wave-uniform base, the lane's row and
         // chunk a 32-bit offset.
-        // g2_swz(16 rb16 + lane / 4) = g(lane) ^ 2 (rb16 & 1), so the lane's
-        // chunk, hence its offset, takes one of two values
+        // g2_swz(16 rb16 + lane / 4) = (lane >> 4) & 3 for every rb16, so the
+        // lane's chunk, hence its offset, is the same for every DMA
         const int8_t *sbase[Sh::DMA_PER_WAVE];
         uint32_t dst[Sh::DMA_PER_WAVE];
-        bool odd[Sh::DMA_PER_WAVE];
-        const uint32_t c0 = (lane & 3) ^ (((lane >> 3) ^ (lane >> 5)) & 3);
+        const uint32_t c0 = (lane & 3) ^ ((lane >> 4) & 3);
         // (residue planes in 16-row x 64-byte blocks: row l/4 of a block is 64 bytes in)
-        const uint32_t voff_e = (uint32_t)((lane >> 2) * 64) + 16 * c0, voff_o = voff_e + 16 * (c0 ^ 2) - 16 * c0;
+        const uint32_t voff = (uint32_t)((lane >> 2) * 64) + 16 * c0;
 #pragma unroll
         for (int u = 0; u < Sh::DMA_PER_WAVE; u++) {
             const int idx = wave_u * Sh::DMA_PER_WAVE + u;
@@ -647,12 +654,11 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
             const int s = j / rbn, rb16 = j % rbn, rows = isA ? Sh::M : 128;
             sbase[u] = uniform_ptr(isA ? gA + s * planeA + (i0 + rb16 * 16) * kpad
                                        : gB + s * planeB + (j0 + rb16 * 16) * kpad);
-            odd[u] = rb16 & 1;
             dst[u] = (isA ? 0 : Sh::A_BYTES) + (s * rows + rb16 * 16) * 64;
         }
 #define ISSUE(k_)                                                                                     \
         _Pragma("unroll") for (int u = 0; u < Sh::DMA_PER_WAVE; u++)                                  \
-            dma16(sbase[u] + (int64_t)(k_) * 1024, odd[u] ? voff_o : voff_e,                           \
+            dma16(sbase[u] + (int64_t)(k_) * 1024, voff,                                                \
                   __builtin_amdgcn_readfirstlane(ls0 + ((k_) % STAGES) * Sh::STAGE + dst[u]));
 
         v16i acc[3][2][2];
