@@ -563,6 +563,8 @@ template <int WM> struct G2Shape {
 };
 
 typedef __attribute__((address_space(3))) void lds_void;
+// word 3 of a raw buffer resource on gfx9-family chips (32-bit data format)
+constexpr int BUF_DWORD3 = 0x00020000;
 
 __device__ __forceinline__ uint32_t lds_offset(const void *p) { return (uint32_t)(uintptr_t)(const lds_void *)p; }
 __device__ __forceinline__ int g2_swz(int r) { return (r >> 2) & 3; }
@@ -866,12 +868,27 @@ __device__ __forceinline__ void recon_store(const v16i (&acc)[4][2], const int64
         const int64_t e = e0 + 64 * wave + 32 * h + c32;
         const int nv = min(RECON_ROWS / 4, nw - rb * (RECON_ROWS / 4));   // words of this block below nw
         int64_t carry = 0;
+        if (nv == RECON_ROWS / 4 && e0 + RECON_ENT <= nent) {
+            // the whole block for every entry (e0 is a multiple of RECON_ENT):
+            // word j of entry e at a wave-uniform base + 4 (e % RECON_ENT) +
+            // j RECON_ENT 4, stored with no address arithmetic and no branch
+            const auto rs = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)(Wv + recon_idx((int64_t)rb * (RECON_ROWS / 4), e0, nw)), 0, -1, BUF_DWORD3);
+            const int voff = (int)(e % RECON_ENT) * 4;
 #pragma unroll
-        for (int j = 0; j < RECON_ROWS / 4; j++) {
-            if (j < nv) {
+            for (int j = 0; j < RECON_ROWS / 4; j++) {
                 const int64_t x = wv[j] + carry;   // |wv| < 2^49
                 carry = x >> 32;
-                if (e < nent) Wv[recon_idx(rb * (RECON_ROWS / 4) + j, e, nw)] = (uint32_t)x;
+                __builtin_amdgcn_raw_buffer_store_b32((uint32_t)x, rs, voff, j * RECON_ENT * 4, 0);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < RECON_ROWS / 4; j++) {
+                if (j < nv) {
+                    const int64_t x = wv[j] + carry;
+                    carry = x >> 32;
+                    if (e < nent) Wv[recon_idx(rb * (RECON_ROWS / 4) + j, e, nw)] = (uint32_t)x;
+                }
             }
         }
         if (e < nent) Cb[recon_idx(rb, e, nrb)] = (int32_t)carry;
