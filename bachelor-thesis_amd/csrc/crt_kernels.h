@@ -1072,18 +1072,20 @@ __device__ __forceinline__ void recon_carry_block(int64_t blk, const uint32_t *_
         neg = H < 0;
     }
 
-    const int64_t nlimb = nw / 2, lim = min(cap, nlimb);
+    const int nlimb = nw / 2, lim = (int)min<int64_t>(cap, nlimb);
     int64_t carry = 0;
     int64_t top = -1;
     bool hi_nz = false;
-    for (int64_t c0 = 0; c0 < nlimb; c0 += 16) {
+    for (int c0 = 0; c0 < nlimb; c0 += 16) {
         // unconditional loads (word index clamped, padding threads read entry 0)
-        // so that all 32 are in flight at once; masked afterwards
+        // so that all 32 are in flight at once; masked afterwards.  Word
+        // indices are 32-bit (nw < 2^31), so their arithmetic stays in
+        // scalar integer registers.
         int64_t xw[32];
 #pragma unroll
         for (int j = 0; j < 32; j++) {
-            const int64_t t = min<int64_t>(2 * c0 + j, nw - 1);
-            xw[j] = (int64_t)wp[t * RECON_ENT] - gam * (int64_t)M32[t];
+            const int t = min(2 * c0 + j, nw - 1);
+            xw[j] = (int64_t)wp[(int64_t)t * RECON_ENT] - gam * (int64_t)M32[t];
         }
         // 2 c0 is a multiple of 32 = BW: word 2 c0 takes the carry of block 2 c0 / BW - 1
         const int64_t cin = c0 > 0 ? (int64_t)cp[(int64_t)(2 * c0 / BW - 1) * RECON_ENT] : 0;
