@@ -86,13 +86,10 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(Queue q, const Reduce
     int64_t t;
     while (next_item(q, j, t)) {
         const ReduceJob &b = jobs[j];
-        if (b.resident && b.kb <= 64)
-            reduce_tile_res<S, 1>(t, b.pc, b.cst, b.P, b.nrb, b.kb, b.X, b.Xrs, b.W, b.transpose, b.rows, b.K, b.kpad,
-                                  b.R, b.plane, err);
-        else if (b.resident && b.kb <= 128)
-            reduce_tile_res<S, 2>(t, b.pc, b.cst, b.P, b.nrb, b.kb, b.X, b.Xrs, b.W, b.transpose, b.rows, b.K, b.kpad,
-                                  b.R, b.plane, err);
-        else if (b.resident)
+        // one instantiation for every byte bound (the constant columns past kb
+        // are zero): each inlined tile function adds its own LDS arrays, and a
+        // second or third would cut the workgroups per CU
+        if (b.resident)
             reduce_tile_res<S, 4>(t, b.pc, b.cst, b.P, b.nrb, b.kb, b.X, b.Xrs, b.W, b.transpose, b.rows, b.K, b.kpad,
                                   b.R, b.plane, err);
         else
