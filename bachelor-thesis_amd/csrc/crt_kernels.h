@@ -135,42 +135,53 @@ __device__ __forceinline__ void reduce_tile(int64_t blk,
     const int rb = (int)(blk % nrb);
     const int64_t e0 = (blk / nrb) * 128;
 
-    // the two entries this thread stages
-    const uint64_t *ep[2];
-    int64_t nlimb[2];
+    // the two entries this thread stages: limb base (padding entries read
+    // entry 0 and mask everything) and limb count
+    int64_t base[2];
+    int nl[2];
+    const int lim = (int)min<int64_t>(kb / 8, W - 1);   // limb loads clamped below both bounds
     for (int h = 0; h < 2; h++) {
-        int64_t e = e0 + (tid >> 2) + 64 * h;
-        int64_t r = e / kpad, k = e % kpad;
-        ep[h] = nullptr; nlimb[h] = 0;
-        if (e < nent && r < rows && k < K) {
-            ep[h] = transpose ? X + (k * Xrs + r) * W : X + (r * Xrs + k) * W;
-            int64_t s = (int64_t)ep[h][W - 1];
-            nlimb[h] = s < 0 ? -s : s;
-            if (rb == 0)   // limbs beyond the plan's bit bound must be zero
-                for (int64_t li = kb / 8; li < nlimb[h]; li++)
-                    if (ep[h][li]) { atomicOr(err, DEV_ERR_INPUT); break; }
-        }
+        const int64_t e = e0 + (tid >> 2) + 64 * h, r = e / kpad, k = e % kpad;
+        const bool real = e < nent && r < rows && k < K;
+        base[h] = real ? (transpose ? (k * Xrs + r) * W : (r * Xrs + k) * W) : 0;
+        const int64_t sz = (int64_t)X[base[h] + W - 1];
+        const int64_t a = real ? min(sz < 0 ? -sz : sz, W - 1) : 0;
+        nl[h] = (int)min<int64_t>(a, lim);
+        if (rb == 0)   // limbs beyond the plan's bit bound must be zero
+            for (int64_t li = kb / 8; li < a; li++)
+                if (X[base[h] + li]) { atomicOr(err, DEV_ERR_INPUT); break; }
     }
     const int part = tid & 3;
     v16i acc[2];
     for (int i = 0; i < 2; i++) acc[i] = (v16i){0};
 
+    // the next step's constants and limbs are loaded (unconditionally, limb
+    // index clamped) before this step's MFMAs, and masked at |size| and
+    // biased to byte - 128 when they are stored to LDS
+    v4i an;
+    uint64_t bn[2][2];
+#define RLOAD(k0_)                                                                                    \
+    {                                                                                                 \
+        an = *(const v4i *)(cst + (int64_t)(rb * 64 + (tid >> 2)) * kb + (k0_) + 16 * part);        \
+        _Pragma("unroll") for (int h = 0; h < 2; h++)                                                 \
+        _Pragma("unroll") for (int u = 0; u < 2; u++)                                                 \
+            bn[h][u] = X[base[h] + min(((k0_) + 16 * part) / 8 + u, lim - 1)];                       \
+    }
+    RLOAD(0);
     for (int k0 = 0; k0 < kb; k0 += 64) {
-        v4i a = *(const v4i *)(cst + (int64_t)(rb * 64 + (tid >> 2)) * kb + k0 + 16 * part);
-        uint64_t b[2][2];
-        for (int h = 0; h < 2; h++)
-            for (int u = 0; u < 2; u++) {
-                int64_t li = (k0 + 16 * part) / 8 + u;
-                uint64_t v = (ep[h] && li < nlimb[h]) ? ep[h][li] : 0;
-                b[h][u] = v ^ 0x8080808080808080ull;   // byte - 128
-            }
         __syncthreads();
-        *(v4i *)(LA + (tid >> 2) * LDS_PITCH + 16 * part) = a;
+        *(v4i *)(LA + (tid >> 2) * LDS_PITCH + 16 * part) = an;
+#pragma unroll
         for (int h = 0; h < 2; h++) {
             uint64_t *d = (uint64_t *)(LB + ((tid >> 2) + 64 * h) * LDS_PITCH + 16 * part);
-            d[0] = b[h][0]; d[1] = b[h][1];
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                const int li = (k0 + 16 * part) / 8 + u;
+                d[u] = (li < nl[h] ? bn[h][u] : 0) ^ 0x8080808080808080ull;
+            }
         }
         __syncthreads();
+        if (k0 + 64 < kb) RLOAD(k0 + 64);
 #pragma unroll
         for (int kk = 0; kk < 2; kk++) {
             v4i bf = *(const v4i *)(LB + (wave * 32 + (lane & 31)) * LDS_PITCH + kk * 32 + 16 * (lane >> 5));
@@ -181,6 +192,7 @@ __device__ __forceinline__ void reduce_tile(int64_t blk,
             }
         }
     }
+#undef RLOAD
 
     // epilogue: lane owns entry column e; register 4g+u of row tile rt is
     // row rt*32 + 8g + 4(lane/32) + u = S*q + s of this row block
