@@ -214,7 +214,19 @@ __device__ __forceinline__ void reduce_tile(int64_t blk,
                 const int q = rb * (64 / S) + (rt * 32 + 8 * g + 4 * (lane >> 5)) / S + w;
                 if (q >= P) continue;
                 uint32_t word = 0;
-                if (real) {
+                if (real && S == 2) {
+                    // 32-bit steps: |acc| <= kb 2^14 < 2^28 (S = 2 plans bound
+                    // products below 2^17 bits: kb < 2^14), so acc_1 is reduced first (below 2p) and
+                    // acc_0 + 256 r1 + kofs + off32 + h lies in [0, 2^32); with
+                    // h = floor(p/2), the residue minus h is the centred residue
+                    const PrimeConst &c = pc[q];
+                    const uint32_t mu32 = (uint32_t)(c.mu >> 32), hp = c.p >> 1;
+                    const int32_t negp = -(int32_t)c.p;
+                    const uint32_t r1 = smod32_lazy(acc[rt][4 * g + 2 * w + 1], negp, mu32, c.off32);
+                    const uint32_t u = (uint32_t)(acc[rt][4 * g + 2 * w] + 256 * (int32_t)r1) + c.kofs + c.off32 + hp;
+                    const int32_t x = (int32_t)bred32(u, negp, mu32) - (int32_t)hp;
+                    word = slice2(neg ? -x : x);
+                } else if (real) {
                     const PrimeConst c = pc[q];
                     int64_t v = c.kofs;
 #pragma unroll
