@@ -120,6 +120,38 @@ __host__ __device__ __forceinline__ int64_t res_off(int S, int64_t r, int64_t k,
     return S == 2 ? (r >> 4) * (kpad << 4) + ((k >> 6) << 10) + ((r & 15) << 6) + (k & 63) : r * kpad + k;
 }
 
+// word of entry el in an LO4 row: the 16-byte group of entries el/4 moves to
+// group (el/4) ^ 1 in the upper half of the row, so that the transpose's
+// 16-byte reads of groups 2c and 2c + 1 (c = 0..15) are free of bank conflicts
+__device__ __forceinline__ int lo4_swz(int el) { return el ^ (((el >> 6) & 1) << 2); }
+// The residues of a 64-row x 128-entry tile (row block rb, entries e0 ..
+// e0 + 127, all inside the plane) from LDS to their planes: LO4 holds, for
+// row group qg (rows 4 qg .. 4 qg + 3) and entry el, the four rows' bytes as
+// one dword at LO4[qg * 128 + lo4_swz(el)].  Thread (qg, ch) takes entries
+// 8 ch .. 8 ch + 7; a 4 x 8 byte transpose gives each row an 8-byte segment,
+// and the 16 threads of one qg write 128 consecutive bytes per row.
+template <int S>
+__device__ __forceinline__ void store_res_tile(const uint32_t *LO4, int tid, int64_t e0, int64_t kpad, int rb, int P,
+                                               int8_t *__restrict__ R, int64_t plane)
+{
+    const int qg = tid >> 4, ch = tid & 15;
+    const int64_t eo = res_off(S, (e0 + 8 * ch) / kpad, (e0 + 8 * ch) % kpad, kpad);   // 8 columns, one row
+    const uint4 a = *(const uint4 *)(LO4 + qg * 128 + lo4_swz(8 * ch));
+    const uint4 b = *(const uint4 *)(LO4 + qg * 128 + lo4_swz(8 * ch + 4));
+    const uint32_t d[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int64_t grow = (int64_t)rb * 64 + 4 * qg + k;   // = S*q + s
+        uint32_t lo = 0, hi = 0;
+#pragma unroll
+        for (int x = 0; x < 4; x++) {
+            lo |= ((d[x] >> (8 * k)) & 255u) << (8 * x);
+            hi |= ((d[x + 4] >> (8 * k)) & 255u) << (8 * x);
+        }
+        if (grow < (int64_t)S * P) *(uint2 *)(R + grow * plane + eo) = make_uint2(lo, hi);
+    }
+}
+
 // ------------------------------------------------------------------------
 // 1. reduce.  Output rows S*q+s x entries; tile 64 rows x 128 entries.
 // ------------------------------------------------------------------------
@@ -194,39 +226,45 @@ __device__ __forceinline__ void reduce_tile(int64_t blk,
     }
 #undef RLOAD
 
-    // epilogue: lane owns entry column e; register 4g+u of row tile rt is
-    // row rt*32 + 8g + 4(lane/32) + u = S*q + s of this row block
-    const int64_t e = e0 + wave * 32 + (lane & 31);
-    if (e >= nent) return;
-    int64_t r = e / kpad, k = e % kpad;
-    bool real = r < rows && k < K;
+    // epilogue: lane owns entry el = wave*32 + lane%32; register 4g+u of row
+    // tile rt is row rt*32 + 8g + 4(lane/32) + u = S*q + s of this row block.
+    // The four rows' digits of each (rt, g) form one dword in LDS (in LB,
+    // free once every wave is past its last MFMA), stored as row segments.
+    // The plane holds whole tiles (nent is a multiple of 128), so every entry
+    // of the tile exists; padding entries have no limbs and are stored as 0.
+    const int el = wave * 32 + (lane & 31);
+    const int64_t e = e0 + el, r = e / kpad, k = e % kpad;
+    const bool real = e < nent && r < rows && k < K;
     bool neg = false;
     if (real) {
         const uint64_t *p = transpose ? X + (k * Xrs + r) * W : X + (r * Xrs + k) * W;
         neg = (int64_t)p[W - 1] < 0;
     }
+    uint32_t *const LO4 = (uint32_t *)LB;
+    __syncthreads();
 #pragma unroll
     for (int rt = 0; rt < 2; rt++)
 #pragma unroll
-        for (int g = 0; g < 4; g++)
+        for (int g = 0; g < 4; g++) {
+            uint32_t dw = 0;
 #pragma unroll
             for (int w = 0; w < 4 / S; w++) {
                 const int q = rb * (64 / S) + (rt * 32 + 8 * g + 4 * (lane >> 5)) / S + w;
-                if (q >= P) continue;
                 uint32_t word = 0;
-                if (real && S == 2) {
+                if (real && q < P && S == 2) {
                     // 32-bit steps: |acc| <= kb 2^14 < 2^28 (S = 2 plans bound
-                    // products below 2^17 bits: kb < 2^14), so acc_1 is reduced first (below 2p) and
-                    // acc_0 + 256 r1 + kofs + off32 + h lies in [0, 2^32); with
-                    // h = floor(p/2), the residue minus h is the centred residue
+                    // products below 2^17 bits: kb < 2^14), so acc_1 is reduced
+                    // first (below 2p) and acc_0 + 256 r1 + kofs + off32 + h lies in
+                    // [0, 2^32); with h = floor(p/2), the residue minus h is the
+                    // centred residue
                     const PrimeConst &c = pc[q];
                     const uint32_t mu32 = (uint32_t)(c.mu >> 32), hp = c.p >> 1;
                     const int32_t negp = -(int32_t)c.p;
                     const uint32_t r1 = smod32_lazy(acc[rt][4 * g + 2 * w + 1], negp, mu32, c.off32);
                     const uint32_t u = (uint32_t)(acc[rt][4 * g + 2 * w] + 256 * (int32_t)r1) + c.kofs + c.off32 + hp;
-                    const int32_t x = (int32_t)bred32(u, negp, mu32) - (int32_t)hp;
-                    word = slice2(neg ? -x : x);
-                } else if (real) {
+                    const int32_t xr = (int32_t)bred32(u, negp, mu32) - (int32_t)hp;
+                    word = slice2(neg ? -xr : xr);
+                } else if (real && q < P) {
                     const PrimeConst c = pc[q];
                     int64_t v = c.kofs;
 #pragma unroll
@@ -235,10 +273,12 @@ __device__ __forceinline__ void reduce_tile(int64_t blk,
                     if (neg && res) res = c.p - res;
                     word = slice_s<S>(centre(res, c.p));
                 }
-                int8_t *o = R + (int64_t)q * S * plane + res_off(S, e / kpad, e % kpad, kpad);
-#pragma unroll
-                for (int s = 0; s < S; s++) o[s * plane] = (int8_t)(word >> (8 * s));
+                dw |= word << (16 * w);
             }
+            LO4[(rt * 8 + 2 * g + (lane >> 5)) * 128 + lo4_swz(el)] = dw;
+        }
+    __syncthreads();
+    store_res_tile<S>(LO4, tid, e0, kpad, rb, P, R, plane);
 }
 
 
@@ -251,10 +291,6 @@ __device__ __forceinline__ void reduce_tile(int64_t blk,
 constexpr int RES_KB = 256;
 constexpr int RES_PITCH = RES_KB + 16;
 constexpr int RES_KG = 2;   // K steps whose A fragments are read ahead of their MFMAs
-// word of entry el in an LO4 row: the 16-byte group of entries el/4 moves to
-// group (el/4) ^ 1 in the upper half of the row, so that the transpose's
-// 16-byte reads of groups 2c and 2c + 1 (c = 0..15) are free of bank conflicts
-__device__ __forceinline__ int lo4_swz(int el) { return el ^ (((el >> 6) & 1) << 2); }
 
 template <int S, int KS>
 __device__ __forceinline__ void reduce_tile_res(int64_t blk,
@@ -410,27 +446,7 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
                 LO4[(rt * 8 + 2 * g + (lane >> 5)) * 128 + lo4_swz(el)] = dw;
             }
         __syncthreads();
-        {
-            // thread: rows 4 qg .. 4 qg + 3 of entries 8 ch .. 8 ch + 7; a
-            // 4 x 8 byte transpose gives each row an 8-byte segment, and the
-            // 16 threads of one qg write 128 consecutive bytes per row
-            const int qg = tid >> 4, ch = tid & 15;
-            const int64_t eo = res_off(S, (e0 + 8 * ch) / kpad, (e0 + 8 * ch) % kpad, kpad);   // 8 columns, one row
-            const uint4 a = *(const uint4 *)(LO4 + qg * 128 + lo4_swz(8 * ch));
-            const uint4 b = *(const uint4 *)(LO4 + qg * 128 + lo4_swz(8 * ch + 4));
-            const uint32_t d[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const int64_t grow = (int64_t)rb * 64 + 4 * qg + k;   // = S*q + s
-                uint32_t lo = 0, hi = 0;
-#pragma unroll
-                for (int x = 0; x < 4; x++) {
-                    lo |= ((d[x] >> (8 * k)) & 255u) << (8 * x);
-                    hi |= ((d[x + 4] >> (8 * k)) & 255u) << (8 * x);
-                }
-                if (grow < (int64_t)S * P) *(uint2 *)(R + grow * plane + eo) = make_uint2(lo, hi);
-            }
-        }
+        store_res_tile<S>(LO4, tid, e0, kpad, rb, P, R, plane);
     }
 #undef CLOAD
 #undef PCLOAD
