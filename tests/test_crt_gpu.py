@@ -36,9 +36,10 @@ def product_plan_M(primes):
 
 
 @pytest.mark.parametrize("transpose", [0, 1])
-def test_reduce_kernel(gpu, slices, transpose):
+@pytest.mark.parametrize("bits,cap", [(700, 14), (3000, 50)])   # entry-resident tile, generic tile (> 256 bytes)
+def test_reduce_kernel(gpu, slices, transpose, bits, cap):
     mgx = gpu
-    rows, K, bits, cap = 70, 45, 700, 14
+    rows, K = 70, 45
     vals = rand_ints(rows, K, bits, 3 + transpose)
     vals[0][0], vals[1][1], vals[2][2] = 0, (1 << bits) - 1, -((1 << bits) - 1)
     X = to_entries(vals if not transpose else [list(r) for r in zip(*vals)], cap)
