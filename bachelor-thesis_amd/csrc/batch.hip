@@ -84,6 +84,7 @@ __global__ void reset_counters_kernel(unsigned long long *c, int n)
 // hides its memory latency with many waves).  One entry-resident
 // instantiation serves every byte bound (the constant columns past kb are
 // zero): each inlined tile function adds its own LDS arrays.
+constexpr int BATCH_RES_KB = 256;   // widest entry (bytes) the batch's entry-resident reduce takes
 template <int S>
 __global__ __launch_bounds__(256) void batch_reduce_res_kernel(Queue q, const ReduceJob *jobs, unsigned *err)
 {
@@ -273,7 +274,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             d.rpadA = round_up(p.K, 128); d.rpadB = round_up(p.N, 128);
             d.kpad = round_up(p.M, 64); d.ystride = crt_ystride(p.K * p.N);
             crt[i] = 1;
-            nred[2 * (views[i].kb > RES_KB) + sc] += 2;
+            nred[2 * (views[i].kb > BATCH_RES_KB) + sc] += 2;
             ngem[2 * sc + (d.kpad > (sc ? KCHUNK2 : KCHUNK))]++;
             nrec[sc]++;
             nrec_all++;
@@ -318,7 +319,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             const Dims &d = dims[i];
             const int sc = v.S == 2;
             const int nrb = (int)(v.rows4pad / 64);
-            const bool resident = v.kb <= RES_KB;
+            const bool resident = v.kb <= BATCH_RES_KB;
             for (int t = 0; t < 2; t++) {
                 ReduceJob r;
                 r.pc = v.pc; r.cst = v.cst; r.P = (int)v.P; r.nrb = nrb; r.kb = (int)v.kb; r.transpose = t;

@@ -114,7 +114,8 @@ __global__ __launch_bounds__(256) void crt_reduce_res_kernel(
 template <int S>
 static auto reduce_res_for(int64_t kb) -> decltype(&crt_reduce_res_kernel<S, 4>)
 {
-    return kb <= 64 ? crt_reduce_res_kernel<S, 1> : kb <= 128 ? crt_reduce_res_kernel<S, 2> : crt_reduce_res_kernel<S, 4>;
+    return kb <= 64 ? crt_reduce_res_kernel<S, 1> : kb <= 128 ? crt_reduce_res_kernel<S, 2>
+         : kb <= 256 ? crt_reduce_res_kernel<S, 4> : crt_reduce_res_kernel<S, 8>;
 }
 
 template <bool CHUNKED>

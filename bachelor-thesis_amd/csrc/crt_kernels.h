@@ -288,8 +288,7 @@ __device__ __forceinline__ void reduce_tile(int64_t blk,
 // block over its 128 entries; the residues of a prime block leave through
 // LDS as 8-byte row segments.  KS: 64-byte K steps (kb <= 64 KS; the
 // constant columns kb .. 64 KS are zero).
-constexpr int RES_KB = 256;
-constexpr int RES_PITCH = RES_KB + 16;
+constexpr int RES_KB = 512;   // widest entry (bytes) of the entry-resident tile
 constexpr int RES_KG = 2;   // K steps whose A fragments are read ahead of their MFMAs
 
 template <int S, int KS>
@@ -299,8 +298,9 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
     int64_t kpad, int8_t *__restrict__ R, int64_t plane, unsigned *__restrict__ err)
 {
     static_assert(KS >= 1 && 64 * KS <= RES_KB, "K steps");
+    constexpr int PITCH = 64 * KS + 16;   // LDS row of the constants: 64 KS bytes + 16 padding
     constexpr int K32 = 2 * KS;   // 32-byte MFMA steps
-    __shared__ __attribute__((aligned(16))) int8_t LC[64 * RES_PITCH];
+    __shared__ __attribute__((aligned(16))) int8_t LC[64 * PITCH];
     __shared__ __attribute__((aligned(16))) int8_t LO[64 * 128];
     // S = 2, per sign of the entry and prime of the row block: -p, floor(2^32/p),
     // the offset K and 128 - floor(p/2) (see the epilogue); the negative
@@ -344,23 +344,23 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
             for (int64_t x = kb / 8; x < a; x++)   // limbs beyond the plan's bound must be zero
                 if (X[base + x]) { atomicOr(err, DEV_ERR_INPUT); break; }
     }
-    // the constants of row block rb: 64 rows x kb bytes, at most four 16-byte
+    // the constants of row block rb: 64 rows x kb bytes, at most KS 16-byte
     // chunks per thread, one block ahead in registers
-    v4i cpre[4];
-    int cst_off[4], lc_off[4];
+    v4i cpre[KS];
+    int cst_off[KS], lc_off[KS];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < KS; u++) {
         const int c = tid + 256 * u, row = c / cpe, part = c - row * cpe;
         cst_off[u] = c < 64 * cpe ? row * kb + 16 * part : -1;
-        lc_off[u] = row * RES_PITCH + 16 * part;
+        lc_off[u] = row * PITCH + 16 * part;
     }
     // columns kb .. 64 KS: zero once (the row blocks' copies never write them)
     for (int c = tid; c < 64 * (4 * KS - cpe); c += 256) {
         const int row = c / (4 * KS - cpe), part = cpe + c % (4 * KS - cpe);
-        *(v4i *)(LC + row * RES_PITCH + 16 * part) = (v4i){0, 0, 0, 0};
+        *(v4i *)(LC + row * PITCH + 16 * part) = (v4i){0, 0, 0, 0};
     }
 #define CLOAD(rb_)                                                                                    \
-    _Pragma("unroll") for (int u = 0; u < 4; u++)                                                     \
+    _Pragma("unroll") for (int u = 0; u < KS; u++)                                                     \
         if (cst_off[u] >= 0) cpre[u] = *(const v4i *)(cst + (int64_t)(rb_) * 64 * kb + cst_off[u]);
     // S = 2: the row block's 32 primes' epilogue constants (thread 32 sg + j:
     // sign sg, prime j), one block ahead in registers
@@ -377,7 +377,7 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
     for (int rb = 0; rb < nrb; rb++) {
         __syncthreads();   // LC, LO and s_pc free again
 #pragma unroll
-        for (int u = 0; u < 4; u++)
+        for (int u = 0; u < KS; u++)
             if (cst_off[u] >= 0) *(v4i *)(LC + lc_off[u]) = cpre[u];
         if (S == 2 && tid < 64) s_pc[(tid >> 5) * 33 + (tid & 31)] = pcv;
         __syncthreads();
@@ -395,7 +395,7 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
             for (int k32 = k0; k32 < k0 + RES_KG && k32 < K32; k32++)
 #pragma unroll
                 for (int rt = 0; rt < 2; rt++)
-                    af[k32 - k0][rt] = *(const v4i *)(LC + (rt * 32 + (lane & 31)) * RES_PITCH + k32 * 32 + 16 * hh);
+                    af[k32 - k0][rt] = *(const v4i *)(LC + (rt * 32 + (lane & 31)) * PITCH + k32 * 32 + 16 * hh);
 #pragma unroll
             for (int k32 = k0; k32 < k0 + RES_KG && k32 < K32; k32++)
 #pragma unroll
@@ -412,19 +412,26 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
             for (int g = 0; g < 4; g++) {
                 uint32_t dw = 0;
                 if (S == 2) {
-                    // |acc| <= 2^22 (kb <= 256) and v = acc_0 + 256 acc_1 is
-                    // congruent to x - 128 sum 2^(8b) (to -x + 127 sum 2^(8b) for a
-                    // negative entry), so u = v + K with K = (128 resp. -127) sum
-                    // 2^(8b) mod p + off32 + h lies in [0, 2^32) and is congruent
-                    // to x + h; with h = floor(p/2), (u mod p) - h is the centred
-                    // residue and (u mod p) + 128 - h its two digits' bytes plus
-                    // 128.  Padding entries have no limbs and come out 0.
+                    // |acc| <= kb 2^14 and v = acc_0 + 256 acc_1 is congruent to
+                    // x - 128 sum 2^(8b) (to -x + 127 sum 2^(8b) for a negative
+                    // entry).  For kb <= 256, |v| < 2^31 - 2^22; wider entries first
+                    // reduce acc_1 below 2p (plus p floor(floor(2^32/p)/2), a
+                    // multiple of p in (2^31 - 2p, 2^31]).  u = v + K with K =
+                    // (128 resp. -127) sum 2^(8b) mod p + off32 + h then lies in
+                    // [0, 2^32) and is congruent to x + h; with h = floor(p/2),
+                    // (u mod p) - h is the centred residue and (u mod p) + 128 - h
+                    // its two digits' bytes plus 128.  Padding entries have no limbs
+                    // and come out 0.
                     const uint4 *tab = s_pc + (neg ? 33 : 0) + 16 * rt + 4 * g + 2 * hh;
                     uint32_t xw[2];
 #pragma unroll
                     for (int w = 0; w < 2; w++) {
                         const uint4 k4 = tab[w];   // -p, mu32, K, 128 - h
-                        const uint32_t u = (uint32_t)(acc[rt][4 * g + 2 * w] + acc[rt][4 * g + 2 * w + 1] * 256) + k4.z;
+                        int32_t hi = acc[rt][4 * g + 2 * w + 1];
+                        if (KS > 4)
+                            hi = (int32_t)bred32_lazy((uint32_t)hi + __umul24(k4.y >> 1, (uint32_t)-(int32_t)k4.x),
+                                                      (int32_t)k4.x, k4.y);
+                        const uint32_t u = (uint32_t)(acc[rt][4 * g + 2 * w] + hi * 256) + k4.z;
                         xw[w] = bred32(u, (int32_t)k4.x, k4.y) + k4.w;
                     }
                     dw = ((xw[0] & 0xffffu) | (xw[1] << 16)) ^ 0x00800080u;

@@ -36,7 +36,7 @@ def product_plan_M(primes):
 
 
 @pytest.mark.parametrize("transpose", [0, 1])
-@pytest.mark.parametrize("bits,cap", [(700, 14), (3000, 50)])   # entry-resident tile, generic tile (> 256 bytes)
+@pytest.mark.parametrize("bits,cap", [(700, 14), (3000, 50), (5000, 82)])   # entry-resident (2 sizes), generic (> 512 bytes)
 def test_reduce_kernel(gpu, slices, transpose, bits, cap):
     mgx = gpu
     rows, K = 70, 45
