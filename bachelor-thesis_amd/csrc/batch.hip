@@ -81,22 +81,25 @@ __global__ void reset_counters_kernel(unsigned long long *c, int n)
 
 // Entry-resident and generic reduce jobs run in kernels of their own, so
 // that each gets its own occupancy (the generic tile needs few registers and
-// hides its memory latency with many waves).  One entry-resident
-// instantiation serves every byte bound (the constant columns past kb are
-// zero): each inlined tile function adds its own LDS arrays.
-constexpr int BATCH_RES_KB = 256;   // widest entry (bytes) the batch's entry-resident reduce takes
-template <int S>
+// hides its memory latency with many waves).  Entry-resident jobs run in
+// two kernels, entries up to 256 bytes and up to 512 bytes, each serving
+// every byte bound below its own (the constant columns past kb are zero):
+// each inlined tile function adds its own LDS arrays and registers.
+template <int S, int KS>
 __global__ __launch_bounds__(256) void batch_reduce_res_kernel(Queue q, const ReduceJob *jobs, unsigned *err)
 {
     int j;
     int64_t t;
     while (next_item(q, j, t)) {
         const ReduceJob &b = jobs[j];
-        reduce_tile_res<S, 4>(t, b.pc, b.cst, b.P, b.nrb, b.kb, b.X, b.Xrs, b.W, b.transpose, b.rows, b.K, b.kpad,
-                              b.R, b.plane, err);
+        reduce_tile_res<S, KS>(t, b.pc, b.cst, b.P, b.nrb, b.kb, b.X, b.Xrs, b.W, b.transpose, b.rows, b.K, b.kpad,
+                               b.R, b.plane, err);
         __syncthreads();
     }
 }
+
+// reduce tile kind for a byte bound: 0 entry-resident up to 256 bytes, 1 generic, 2 entry-resident up to 512
+inline int reduce_kind(int64_t kb) { return kb <= 256 ? 0 : kb <= RES_KB ? 2 : 1; }
 
 template <int S>
 __global__ __launch_bounds__(256) void batch_reduce_gen_kernel(Queue q, const ReduceJob *jobs, unsigned *err)
@@ -247,15 +250,16 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
     if (st != MGX_OK) { delete b; return st; }
 
     // ---- job tables (sizes first, then one scratch block, then real pointers).
-    // Classes: reduce by slice count (sc = 1 for S = 2) and tile (2 generic +
-    // sc), reconstruct-GEMM by slice count, residue GEMM by slice count and
+    // Classes: reduce by slice count (sc = 1 for S = 2) and tile (2 kind + sc;
+    // kind 0: entries up to 256 bytes, 1: generic, 2: up to 512 bytes),
+    // reconstruct-GEMM by slice count, residue GEMM by slice count and
     // chunking (2 sc + chunked); the carry pass takes every CRT product in one
     // queue.
-    std::vector<ReduceJob> rj[4];
+    std::vector<ReduceJob> rj[6];
     std::vector<GemmJob> gj[4];
     std::vector<ReconJob> cj[2], kj;
     std::vector<PlainJob> pj[NCLASS];
-    std::vector<int64_t> rp[4], gp[4], cp[2], kp{0}, pp[NCLASS];
+    std::vector<int64_t> rp[6], gp[4], cp[2], kp{0}, pp[NCLASS];
     for (auto &v : rp) v.push_back(0);
     for (auto &v : gp) v.push_back(0);
     for (auto &v : cp) v.push_back(0);
@@ -263,7 +267,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
     struct Dims { int64_t rpadA, rpadB, kpad, ystride; };
     std::vector<Dims> dims(count);
     std::vector<int> pclass(count, -1), crt(count, 0);
-    size_t nred[4] = {0, 0, 0, 0}, ngem[4] = {0, 0, 0, 0}, nrec[2] = {0, 0}, nrec_all = 0;
+    size_t nred[6] = {0, 0, 0, 0, 0, 0}, ngem[4] = {0, 0, 0, 0}, nrec[2] = {0, 0}, nrec_all = 0;
     size_t npl[NCLASS] = {0, 0, 0, 0, 0, 0};
     for (int64_t i = 0; i < count; i++) {
         const mgx_product &p = prods[i];
@@ -274,7 +278,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             d.rpadA = round_up(p.K, 128); d.rpadB = round_up(p.N, 128);
             d.kpad = round_up(p.M, 64); d.ystride = crt_ystride(p.K * p.N);
             crt[i] = 1;
-            nred[2 * (views[i].kb > BATCH_RES_KB) + sc] += 2;
+            nred[2 * reduce_kind(views[i].kb) + sc] += 2;
             ngem[2 * sc + (d.kpad > (sc ? KCHUNK2 : KCHUNK))]++;
             nrec[sc]++;
             nrec_all++;
@@ -299,8 +303,8 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
         o_ga[i] = L.take(ceil_div(4 * v.nw, RECON_ROWS) * d.ystride * 12);   // G (int64), then Cb (int32)
     }
     const int64_t o_ctr = L.take(64 * 8);
-    int64_t o_rj[4], o_rp[4], o_gj[4], o_gp[4], o_cj[2], o_cp[2], o_pj[NCLASS], o_pp[NCLASS];
-    for (int c = 0; c < 4; c++) { o_rj[c] = L.take((int64_t)(nred[c] * sizeof(ReduceJob))); o_rp[c] = L.take((int64_t)(nred[c] + 1) * 8); }
+    int64_t o_rj[6], o_rp[6], o_gj[4], o_gp[4], o_cj[2], o_cp[2], o_pj[NCLASS], o_pp[NCLASS];
+    for (int c = 0; c < 6; c++) { o_rj[c] = L.take((int64_t)(nred[c] * sizeof(ReduceJob))); o_rp[c] = L.take((int64_t)(nred[c] + 1) * 8); }
     for (int c = 0; c < 4; c++) { o_gj[c] = L.take((int64_t)(ngem[c] * sizeof(GemmJob))); o_gp[c] = L.take((int64_t)(ngem[c] + 1) * 8); }
     for (int c = 0; c < 2; c++) { o_cj[c] = L.take((int64_t)(nrec[c] * sizeof(ReconJob))); o_cp[c] = L.take((int64_t)(nrec[c] + 1) * 8); }
     const int64_t o_kj = L.take((int64_t)(nrec_all * sizeof(ReconJob))), o_kp = L.take((int64_t)(nrec_all + 1) * 8);
@@ -319,7 +323,8 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             const Dims &d = dims[i];
             const int sc = v.S == 2;
             const int nrb = (int)(v.rows4pad / 64);
-            const bool resident = v.kb <= BATCH_RES_KB;
+            const int kind = reduce_kind(v.kb);
+            const bool resident = kind != 1;
             for (int t = 0; t < 2; t++) {
                 ReduceJob r;
                 r.pc = v.pc; r.cst = v.cst; r.P = (int)v.P; r.nrb = nrb; r.kb = (int)v.kb; r.transpose = t;
@@ -328,7 +333,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
                 r.rows = t ? p.N : p.K; r.K = p.M; r.kpad = d.kpad;
                 r.nent = (t ? d.rpadB : d.rpadA) * d.kpad; r.plane = r.nent;
                 r.R = (int8_t *)(base + (t ? o_rb[i] : o_ra[i]));
-                const int rc = 2 * !resident + sc;
+                const int rc = 2 * kind + sc;
                 rj[rc].push_back(r);
                 rp[rc].push_back(rp[rc].back() + ceil_div(r.nent, 128) * (resident ? 1 : nrb));
             }
@@ -366,7 +371,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
         if (n) MGX_HIP(hipMemcpy(base + o, src, n, hipMemcpyHostToDevice));
         return MGX_OK;
     };
-    for (int c = 0; c < 4 && st == MGX_OK; c++) {
+    for (int c = 0; c < 6 && st == MGX_OK; c++) {
         st = up(o_rj[c], rj[c].data(), rj[c].size() * sizeof(ReduceJob));
         if (st == MGX_OK) st = up(o_rp[c], rp[c].data(), rp[c].size() * 8);
     }
@@ -387,8 +392,8 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
     if (st != MGX_OK) { mgx_batch_destroy(b); return st; }
 
     // ---- capture: reset counters, plain classes, reduce, residue GEMM, reconstruct (GEMM, carry).
-    // Counter slots: 0-1 entry-resident reduce, 2-5 residue GEMM, 6-7 reconstruct GEMM, 8 carry, 9-14 plain
-    // classes, 15-16 generic reduce.
+    // Counter slots: 0-1 entry-resident reduce (256 bytes), 2-5 residue GEMM, 6-7 reconstruct GEMM, 8 carry,
+    // 9-14 plain classes, 15-16 generic reduce, 17-18 entry-resident reduce (512 bytes).
     unsigned long long *ctr = (unsigned long long *)(base + o_ctr);
     unsigned *err = device_error_word();
     const int toom = (int)param(MGX_PARAM_TOOM22);
@@ -418,13 +423,15 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             default: launch_plain<8>(grid, cs, q, jobs, toom, err); break;
             }
         }
-        for (int c = 0; c < 4; c++) {
+        for (int c = 0; c < 6; c++) {
             if (rj[c].empty()) continue;
-            const bool gen = c >= 2, two = c & 1;
-            int grid = (int)std::min<int64_t>(rp[c].back(), gen ? 2048 : 1024);
-            auto kern = gen ? (two ? batch_reduce_gen_kernel<2> : batch_reduce_gen_kernel<4>)
-                            : (two ? batch_reduce_res_kernel<2> : batch_reduce_res_kernel<4>);
-            hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, cs, queue(gen ? 15 + two : c, o_rp[c], rj[c].size()),
+            const int kind = c >> 1, two = c & 1;
+            int grid = (int)std::min<int64_t>(rp[c].back(), kind == 1 ? 2048 : 1024);
+            auto kern = kind == 1 ? (two ? batch_reduce_gen_kernel<2> : batch_reduce_gen_kernel<4>)
+                      : kind == 2 ? (two ? batch_reduce_res_kernel<2, 8> : batch_reduce_res_kernel<4, 8>)
+                                  : (two ? batch_reduce_res_kernel<2, 4> : batch_reduce_res_kernel<4, 4>);
+            const int slot = kind == 0 ? two : kind == 1 ? 15 + two : 17 + two;
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, cs, queue(slot, o_rp[c], rj[c].size()),
                                (const ReduceJob *)(base + o_rj[c]), err);
         }
         for (int c = 0; c < 4; c++) {
