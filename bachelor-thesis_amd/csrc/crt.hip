@@ -440,7 +440,7 @@ int mgx_crt_reduce(const mgx_crt_plan *pl, int transpose, int64_t rows, int64_t 
     mgx_crt_residue_dims(pl, rows, K, &rpad, &kpad);
     int64_t nent = rpad * kpad, plane = rpad * kpad;
     int nrb = (int)(pl->rows4pad / 64);
-    const bool resident = pl->kb <= RES_KB;   // the entries' bytes fit LDS: read them once
+    const bool resident = pl->kb <= RES_KB;   // the entries' bytes fit registers: read them once
     int64_t nblk = ceil_div(nent, 128) * (resident ? 1 : nrb);
     if (nblk > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "matrix too large for one reduce launch");
     auto res = pl->S == 2 ? reduce_res_for<2>(pl->kb) : reduce_res_for<4>(pl->kb);
