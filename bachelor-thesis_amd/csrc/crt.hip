@@ -150,13 +150,13 @@ template <int S, bool RESIDENT>
 __global__ __launch_bounds__(256, 2) void crt_recon_gemm_kernel(
     const PrimeConst *__restrict__ pc, int P, const int8_t *__restrict__ Mc, int Kr, int nrb, int nw,
     const uint32_t *__restrict__ Y, int64_t ystride, int64_t nent, uint32_t *__restrict__ Wv,
-    int32_t *__restrict__ Cb, int64_t *__restrict__ G, int64_t total)
+    int32_t *__restrict__ Cb, int64_t *__restrict__ G, int64_t total, int epw)
 {
     // XCD-aware order: the row blocks of one entry block share an L2
     int64_t per = (total + 7) / 8;
     int64_t v = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
     if (v >= total) return;
-    recon_gemm_tile<S, RESIDENT>(v, pc, P, Mc, Kr, nrb, nw, Y, ystride, nent, Wv, Cb, G);
+    recon_gemm_tile<S, RESIDENT>(v, pc, P, Mc, Kr, nrb, nw, Y, ystride, nent, Wv, Cb, G, epw);
 }
 
 __global__ __launch_bounds__(256) void crt_recon_carry_kernel(
@@ -509,14 +509,15 @@ int reconstruct_with(const mgx_crt_plan *pl, int64_t K, int64_t N, const uint32_
     int64_t *G = (int64_t *)(scr + (pl->nw + nrb) * ystride * 4);
     const int64_t lds = recon_res_lds(pl->Kr, pl->S);
     const bool resident = lds <= RECON_RES_MAX_LDS;
-    // resident workgroups sweep RECON_EPW entry blocks each
-    const int64_t total = ceil_div(nent, RECON_ENT * (resident ? RECON_EPW : 1)) * nrb;
+    // resident workgroups sweep epw entry blocks each
+    const int epw = resident ? recon_epw(nent, nrb) : 1;
+    const int64_t total = ceil_div(nent, RECON_ENT * epw) * nrb;
     if (round_up(total, 8) > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "product too large for one reconstruct launch");
     auto kern = pl->S == 2 ? (resident ? crt_recon_gemm_kernel<2, true> : crt_recon_gemm_kernel<2, false>)
                            : (resident ? crt_recon_gemm_kernel<4, true> : crt_recon_gemm_kernel<4, false>);
     if (resident) MGX_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(kern, dim3((unsigned)round_up(total, 8)), dim3(256), resident ? (unsigned)lds : 0u, s, pl->d_pc,
-                       (int)pl->P, pl->d_mc, (int)pl->Kr, nrb, (int)pl->nw, Y, ystride, nent, Wv, Cb, G, total);
+                       (int)pl->P, pl->d_mc, (int)pl->Kr, nrb, (int)pl->nw, Y, ystride, nent, Wv, Cb, G, total, epw);
     MGX_HIP(hipGetLastError());
     hipLaunchKernelGGL(crt_recon_carry_kernel, dim3((unsigned)ceil_div(nent, CARRY_ENT)), dim3(256), 0, s, Wv, Cb,
                        ystride, (int)pl->nw, nrb, G, pl->S == 2 ? 45 : 32, pl->d_m32, nent, N, (uint64_t *)C, Crs,

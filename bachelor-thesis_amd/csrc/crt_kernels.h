@@ -864,10 +864,16 @@ constexpr int RECON_ROWS = 128;   // result bytes per workgroup
 constexpr int RECON_ENT = 256;    // entries per workgroup pass (64 per wave)
 // RESIDENT: the workgroup's whole row block of Mc (128 x Kr bytes) and (S = 2)
 // floor(2^45/p) of every prime sit in dynamic LDS, loaded once, and the K loop runs without
-// barriers; the workgroup then sweeps RECON_EPW consecutive entry blocks with
-// them.  Used while that fits beside a second workgroup.
+// barriers; the workgroup then sweeps epw consecutive entry blocks with
+// them (recon_epw: more for larger products, so that fewer workgroups load
+// the block while every CU still gets several).  Used while that fits beside
+// a second workgroup.
 constexpr int RECON_RES_MAX_LDS = 144 * 1024;
-constexpr int RECON_EPW = 4;
+inline int recon_epw(int64_t nent, int nrb)
+{
+    const int64_t blocks = (nent + RECON_ENT - 1) / RECON_ENT;
+    return blocks * nrb >= 8 * 2048 ? 8 : blocks * nrb >= 4 * 1024 ? 4 : 2;
+}
 // Wv, Cb and G are stored in blocks of RECON_ENT entries: index t of entry e
 // (t < nt) at (e / RECON_ENT) nt RECON_ENT + t RECON_ENT + e % RECON_ENT, so
 // that a reconstruct workgroup's words and the carry pass's reads are long
@@ -966,7 +972,7 @@ __device__ __forceinline__ int64_t gamma_term(uint32_t wd, int sub, int q, int P
 template <int S, bool RESIDENT>
 __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *__restrict__ pc, int P,
     const int8_t *__restrict__ Mc, int Kr, int nrb, int nw, const uint32_t *__restrict__ Y, int64_t ystride,
-    int64_t nent, uint32_t *__restrict__ Wv, int32_t *__restrict__ Cb, int64_t *__restrict__ G)
+    int64_t nent, uint32_t *__restrict__ Wv, int32_t *__restrict__ Cb, int64_t *__restrict__ G, int epw = 1)
 {
     __shared__ __attribute__((aligned(16))) int8_t LAs[RESIDENT ? 16 : RECON_ROWS * LDS_PITCH];
     extern __shared__ __attribute__((aligned(16))) int8_t LDYN[];
@@ -975,7 +981,7 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
     int32_t *const s_f45 = (int32_t *)(LDYN + RECON_ROWS * (Kr + 16));   // RESIDENT only
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, h = lane >> 5, c32 = lane & 31;
     const int rb = (int)(blk % nrb);
-    const int64_t eb0 = (blk / nrb) * (RESIDENT ? RECON_EPW : 1);
+    const int64_t eb0 = (blk / nrb) * (RESIDENT ? epw : 1);
 
     if (RESIDENT) {
         // 128 rows x Kr bytes in 16-byte chunks, eight loads in flight per thread
@@ -998,7 +1004,7 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
         __syncthreads();
     }
 
-    for (int64_t eb = eb0; eb < eb0 + (RESIDENT ? RECON_EPW : 1); eb++) {
+    for (int64_t eb = eb0; eb < eb0 + (RESIDENT ? epw : 1); eb++) {
     const int64_t e0 = eb * RECON_ENT;
     if (e0 >= nent) break;
     const uint32_t *yp = Y + e0 + 64 * wave + c32;
