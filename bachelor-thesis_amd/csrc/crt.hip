@@ -102,13 +102,24 @@ __global__ __launch_bounds__(256) void crt_reduce_kernel(
     reduce_tile<S>(blockIdx.x, pc, cst, P, nrb, kb, X, Xrs, W, transpose, rows, K, kpad, nent, R, plane, err);
 }
 
+// one operand of an entry-resident reduce launch
+struct ResSide {
+    const uint64_t *X;
+    int64_t Xrs, rows, plane;
+    int transpose;
+    int8_t *R;
+};
+// Up to two operands (A and B of one product share the plan, K and kpad) in
+// one launch: workgroups [0, nblk0) reduce side 0, the rest side 1.
 template <int S, int KS>
 __global__ __launch_bounds__(256) void crt_reduce_res_kernel(
-    const PrimeConst *__restrict__ pc, const int8_t *__restrict__ cst, int P, int nrb, int kb,
-    const uint64_t *__restrict__ X, int64_t Xrs, int64_t W, int transpose, int64_t rows, int64_t K,
-    int64_t kpad, int8_t *__restrict__ R, int64_t plane, unsigned *__restrict__ err)
+    const PrimeConst *__restrict__ pc, const int8_t *__restrict__ cst, int P, int nrb, int kb, int64_t W, int64_t K,
+    int64_t kpad, ResSide s0, ResSide s1, int64_t nblk0, unsigned *__restrict__ err)
 {
-    reduce_tile_res<S, KS>(blockIdx.x, pc, cst, P, nrb, kb, X, Xrs, W, transpose, rows, K, kpad, R, plane, err);
+    const bool second = (int64_t)blockIdx.x >= nblk0;
+    const ResSide &sd = second ? s1 : s0;
+    reduce_tile_res<S, KS>(second ? blockIdx.x - nblk0 : blockIdx.x, pc, cst, P, nrb, kb, sd.X, sd.Xrs, W,
+                           sd.transpose, sd.rows, K, kpad, sd.R, sd.plane, err);
 }
 // the entry-resident reduce for a plan's byte bound kb (<= RES_KB, a multiple of 64)
 template <int S>
@@ -445,11 +456,11 @@ int mgx_crt_reduce(const mgx_crt_plan *pl, int transpose, int64_t rows, int64_t 
     if (nblk > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "matrix too large for one reduce launch");
     auto res = pl->S == 2 ? reduce_res_for<2>(pl->kb) : reduce_res_for<4>(pl->kb);
     auto gen = pl->S == 2 ? crt_reduce_kernel<2> : crt_reduce_kernel<4>;
-    if (resident)
+    if (resident) {
+        const ResSide sd{(const uint64_t *)X, Xrs, rows, plane, transpose, R};
         hipLaunchKernelGGL(res, dim3((unsigned)nblk), dim3(256), 0, as_stream(stream), pl->d_pc, pl->d_cst,
-                           (int)pl->P, nrb, (int)pl->kb, (const uint64_t *)X, Xrs, cap + 1, transpose, rows, K, kpad,
-                           R, plane, device_error_word());
-    else
+                           (int)pl->P, nrb, (int)pl->kb, cap + 1, K, kpad, sd, sd, nblk, device_error_word());
+    } else
         hipLaunchKernelGGL(gen, dim3((unsigned)nblk), dim3(256), 0, as_stream(stream), pl->d_pc, pl->d_cst,
                            (int)pl->P, nrb, (int)pl->kb, (const uint64_t *)X, Xrs, cap + 1, transpose, rows, K, kpad,
                            nent, R, plane, device_error_word());
@@ -629,8 +640,22 @@ int mgx_crt_mfma_matmul(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t 
         int8_t *RA = (int8_t *)scr, *RB = (int8_t *)scr + bytesA;
         uint32_t *Y = (uint32_t *)(scr + offY);
 
-        st = mgx_crt_reduce(pl, 0, K, M, A, Ars, cap, 0, RA, stream);
-        if (st == MGX_OK) st = mgx_crt_reduce(pl, 1, N, M, B, Brs, cap, 0, RB, stream);
+        if (pl->kb <= RES_KB) {   // both operands in one launch: one tail instead of two
+            const int64_t nA = rpadA * kpad, nB = rpadB * kpad;
+            const ResSide sa{(const uint64_t *)A, Ars, K, nA, 0, RA}, sb{(const uint64_t *)B, Brs, N, nB, 1, RB};
+            auto res = pl->S == 2 ? reduce_res_for<2>(pl->kb) : reduce_res_for<4>(pl->kb);
+            const int64_t nblk = ceil_div(nA, 128) + ceil_div(nB, 128);
+            if (nblk > 0x7fffffffll) st = fail(MGX_ERR_SHAPE, "matrices too large for one reduce launch");
+            else {
+                hipLaunchKernelGGL(res, dim3((unsigned)nblk), dim3(256), 0, s, pl->d_pc, pl->d_cst, (int)pl->P,
+                                   (int)(pl->rows4pad / 64), (int)pl->kb, cap + 1, M, kpad, sa, sb,
+                                   ceil_div(nA, 128), device_error_word());
+                if (hipGetLastError() != hipSuccess) st = fail(MGX_ERR_HIP, "reduce launch");
+            }
+        } else {
+            st = mgx_crt_reduce(pl, 0, K, M, A, Ars, cap, 0, RA, stream);
+            if (st == MGX_OK) st = mgx_crt_reduce(pl, 1, N, M, B, Brs, cap, 0, RB, stream);
+        }
         if (st == MGX_OK) st = mgx_crt_residue_gemm(pl, K, M, N, RA, RB, Y, stream);
         if (st == MGX_OK) st = reconstruct_with(pl, K, N, Y, C, Crs, cap, scr + offR, s);
         scratch_free(scr, s);
