@@ -68,7 +68,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path, so --gpus must be 1",
+              file=sys.stderr)
         sys.exit(2)
     if args.batch:
         return run_batch(args)
