@@ -1138,7 +1138,7 @@ __device__ __forceinline__ void recon_carry_block(int64_t blk, const uint32_t *_
 #pragma unroll
         for (int j = 0; j < 32; j++) {
             const int t = min(2 * c0 + j, nw - 1);
-            xw[j] = (int64_t)wp[(int64_t)t * RECON_ENT] - gam * (int64_t)M32[t];
+            xw[j] = (int64_t)__builtin_nontemporal_load(wp + (int64_t)t * RECON_ENT) - gam * (int64_t)M32[t];
         }
         // 2 c0 is a multiple of 32 = BW: word 2 c0 takes the carry of block 2 c0 / BW - 1
         const int64_t cin = c0 > 0 ? (int64_t)cp[(int64_t)(2 * c0 / BW - 1) * RECON_ENT] : 0;
