@@ -924,7 +924,8 @@ __device__ __forceinline__ void recon_store(const v16i (&acc)[4][2], const int64
         if (nv == RECON_ROWS / 4 && e0 + RECON_ENT <= nent) {
             // the whole block for every entry (e0 is a multiple of RECON_ENT):
             // word j of entry e at a wave-uniform base + 4 (e % RECON_ENT) +
-            // j RECON_ENT 4, stored with no address arithmetic and no branch
+            // j RECON_ENT 4, stored with no address arithmetic and no branch;
+            // cache policy 2 (non-temporal): the carry pass reads each word once
             const auto rs = __builtin_amdgcn_make_buffer_rsrc(
                 (void *)(Wv + recon_idx((int64_t)rb * (RECON_ROWS / 4), e0, nw)), 0, -1, BUF_DWORD3);
             const int voff = (int)(e % RECON_ENT) * 4;
@@ -932,7 +933,7 @@ __device__ __forceinline__ void recon_store(const v16i (&acc)[4][2], const int64
             for (int j = 0; j < RECON_ROWS / 4; j++) {
                 const int64_t x = wv[j] + carry;   // |wv| < 2^49
                 carry = x >> 32;
-                __builtin_amdgcn_raw_buffer_store_b32((uint32_t)x, rs, voff, j * RECON_ENT * 4, 0);
+                __builtin_amdgcn_raw_buffer_store_b32((uint32_t)x, rs, voff, j * RECON_ENT * 4, 2);
             }
         } else {
 #pragma unroll
