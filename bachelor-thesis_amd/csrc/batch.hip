@@ -19,7 +19,7 @@ namespace mgx {
 struct ReduceJob {
     const PrimeConst *pc;
     const int8_t *cst;
-    int P, nrb, kb, transpose, resident;
+    int P, nrb, kb, xbits, transpose, resident;
     const uint64_t *X;
     int64_t Xrs, W, rows, K, kpad, nent, plane;
     int8_t *R;
@@ -92,7 +92,8 @@ __global__ __launch_bounds__(256) void batch_reduce_res_kernel(Queue q, const Re
     int64_t t;
     while (next_item(q, j, t)) {
         const ReduceJob &b = jobs[j];
-        reduce_tile_res<S, KS>(t, b.pc, b.cst, b.P, b.nrb, b.kb, b.X, b.Xrs, b.W, b.transpose, b.rows, b.K, b.kpad,
+        reduce_tile_res<S, KS>(t, b.pc, b.cst, b.P, b.nrb, b.kb, b.xbits, b.X, b.Xrs, b.W, b.transpose, b.rows, b.K,
+                               b.kpad,
                                b.R, b.plane, err);
         __syncthreads();
     }
@@ -108,7 +109,8 @@ __global__ __launch_bounds__(256) void batch_reduce_gen_kernel(Queue q, const Re
     int64_t t;
     while (next_item(q, j, t)) {
         const ReduceJob &b = jobs[j];
-        reduce_tile<S>(t, b.pc, b.cst, b.P, b.nrb, b.kb, b.X, b.Xrs, b.W, b.transpose, b.rows, b.K, b.kpad, b.nent,
+        reduce_tile<S>(t, b.pc, b.cst, b.P, b.nrb, b.kb, b.xbits, b.X, b.Xrs, b.W, b.transpose, b.rows, b.K, b.kpad,
+                       b.nent,
                        b.R, b.plane, err);
         __syncthreads();
     }
@@ -181,6 +183,7 @@ using namespace mgx;
 
 struct mgx_batch {
     std::vector<int> routes;
+    std::vector<const mgx_crt_plan *> plans;   // pinned in the plan cache while the batch lives
     void *scratch = nullptr;       // residues, Y, column sums, job tables, prefixes, counters
     int64_t scratch_bytes = 0;
     hipGraph_t graph = nullptr;
@@ -244,10 +247,13 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
         if (st == MGX_OK && r == MGX_ROUTE_CRT) {
             const mgx_crt_plan *pl = nullptr;
             st = cached_plan(p.M, bitsA[i], bitsB[i], &pl);
-            if (st == MGX_OK) st = plan_view(pl, &views[i]);
+            if (st == MGX_OK) {
+                b->plans.push_back(pl);   // the graph reads its tables on every replay
+                st = plan_view(pl, &views[i]);
+            }
         }
     }
-    if (st != MGX_OK) { delete b; return st; }
+    if (st != MGX_OK) { mgx_batch_destroy(b); return st; }
 
     // ---- job tables (sizes first, then one scratch block, then real pointers).
     // Classes: reduce by slice count (sc = 1 for S = 2) and tile (2 kind + sc;
@@ -328,6 +334,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             for (int t = 0; t < 2; t++) {
                 ReduceJob r;
                 r.pc = v.pc; r.cst = v.cst; r.P = (int)v.P; r.nrb = nrb; r.kb = (int)v.kb; r.transpose = t;
+                r.xbits = (int)(t ? v.bitsB : v.bitsA);
                 r.resident = resident;
                 r.X = (const uint64_t *)(t ? p.B : p.A); r.Xrs = t ? p.Brs : p.Ars; r.W = p.cap + 1;
                 r.rows = t ? p.N : p.K; r.K = p.M; r.kpad = d.kpad;
@@ -486,10 +493,9 @@ int mgx_batch_destroy(mgx_batch *b)
     if (!b) return MGX_OK;
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);
-    if (b->scratch) {
-        hipDeviceSynchronize();
-        hipFree(b->scratch);
-    }
+    if (b->scratch || !b->plans.empty()) (void)hipDeviceSynchronize();   // replays still queued
+    if (b->scratch) hipFree(b->scratch);
+    for (const mgx_crt_plan *pl : b->plans) release_plan(pl);
     delete b;
     return MGX_OK;
 }

@@ -95,11 +95,11 @@ __global__ __launch_bounds__(256) void scan_bits_kernel(ScanArgs sa, int64_t W, 
 // ------------------------------------------------------------------------
 template <int S>
 __global__ __launch_bounds__(256) void crt_reduce_kernel(
-    const PrimeConst *__restrict__ pc, const int8_t *__restrict__ cst, int P, int nrb, int kb,
+    const PrimeConst *__restrict__ pc, const int8_t *__restrict__ cst, int P, int nrb, int kb, int xbits,
     const uint64_t *__restrict__ X, int64_t Xrs, int64_t W, int transpose, int64_t rows, int64_t K,
     int64_t kpad, int64_t nent, int8_t *__restrict__ R, int64_t plane, unsigned *__restrict__ err)
 {
-    reduce_tile<S>(blockIdx.x, pc, cst, P, nrb, kb, X, Xrs, W, transpose, rows, K, kpad, nent, R, plane, err);
+    reduce_tile<S>(blockIdx.x, pc, cst, P, nrb, kb, xbits, X, Xrs, W, transpose, rows, K, kpad, nent, R, plane, err);
 }
 
 // one operand of an entry-resident reduce launch
@@ -107,6 +107,7 @@ struct ResSide {
     const uint64_t *X;
     int64_t Xrs, rows, plane;
     int transpose;
+    int bits;   // |entries| < 2^bits (the plan's bound for this operand)
     int8_t *R;
 };
 // Up to two operands (A and B of one product share the plan, K and kpad) in
@@ -118,7 +119,7 @@ __global__ __launch_bounds__(256) void crt_reduce_res_kernel(
 {
     const bool second = (int64_t)blockIdx.x >= nblk0;
     const ResSide &sd = second ? s1 : s0;
-    reduce_tile_res<S, KS>(second ? blockIdx.x - nblk0 : blockIdx.x, pc, cst, P, nrb, kb, sd.X, sd.Xrs, W,
+    reduce_tile_res<S, KS>(second ? blockIdx.x - nblk0 : blockIdx.x, pc, cst, P, nrb, kb, sd.bits, sd.X, sd.Xrs, W,
                            sd.transpose, sd.rows, K, kpad, sd.R, sd.plane, err);
 }
 // the entry-resident reduce for a plan's byte bound kb (<= RES_KB, a multiple of 64)
@@ -237,11 +238,14 @@ std::vector<uint32_t> &prime_cache(int S)
     static std::vector<uint32_t> v2, v4;
     return S == 2 ? v2 : v4;
 }
+// guards both prime caches: plans may be created from several host threads
+std::mutex g_prime_mu;
 
 // primes below the limit for S, descending, until their product exceeds
 // 2^bits; empty when the primes above the floor do not suffice (S = 2)
 std::vector<uint32_t> primes_for_bits(int64_t bits, hb::Nat &M, int S)
 {
+    std::lock_guard<std::mutex> lk(g_prime_mu);
     auto &cache = prime_cache(S);
     std::vector<uint32_t> out;
     M = hb::from_u64(1);
@@ -444,9 +448,10 @@ int mgx_crt_residue_dims(const mgx_crt_plan *pl, int64_t rows, int64_t K, int64_
 int mgx_crt_reduce(const mgx_crt_plan *pl, int transpose, int64_t rows, int64_t K, const mgx_limb *X, int64_t Xrs,
                    int64_t cap, int64_t nbytes, int8_t *R, void *stream)
 {
-    (void)nbytes;
     if (!pl || !X || !R) return fail(MGX_ERR_ARG, "null argument");
-    if ((int64_t)8 * cap < pl->kb && false) return fail(MGX_ERR_SHAPE, "cap");
+    if (nbytes != 0) return fail(MGX_ERR_ARG, "nbytes is reserved and must be 0");
+    if (rows < 0 || K < 0 || cap < 1) return fail(MGX_ERR_SHAPE, "bad shape");
+    const int xbits = (int)(transpose ? pl->bitsB : pl->bitsA);
     int64_t rpad, kpad;
     mgx_crt_residue_dims(pl, rows, K, &rpad, &kpad);
     int64_t nent = rpad * kpad, plane = rpad * kpad;
@@ -457,12 +462,12 @@ int mgx_crt_reduce(const mgx_crt_plan *pl, int transpose, int64_t rows, int64_t 
     auto res = pl->S == 2 ? reduce_res_for<2>(pl->kb) : reduce_res_for<4>(pl->kb);
     auto gen = pl->S == 2 ? crt_reduce_kernel<2> : crt_reduce_kernel<4>;
     if (resident) {
-        const ResSide sd{(const uint64_t *)X, Xrs, rows, plane, transpose, R};
+        const ResSide sd{(const uint64_t *)X, Xrs, rows, plane, transpose, xbits, R};
         hipLaunchKernelGGL(res, dim3((unsigned)nblk), dim3(256), 0, as_stream(stream), pl->d_pc, pl->d_cst,
                            (int)pl->P, nrb, (int)pl->kb, cap + 1, K, kpad, sd, sd, nblk, device_error_word());
     } else
         hipLaunchKernelGGL(gen, dim3((unsigned)nblk), dim3(256), 0, as_stream(stream), pl->d_pc, pl->d_cst,
-                           (int)pl->P, nrb, (int)pl->kb, (const uint64_t *)X, Xrs, cap + 1, transpose, rows, K, kpad,
+                           (int)pl->P, nrb, (int)pl->kb, xbits, (const uint64_t *)X, Xrs, cap + 1, transpose, rows, K, kpad,
                            nent, R, plane, device_error_word());
     MGX_HIP(hipGetLastError());
     return MGX_OK;
@@ -561,6 +566,13 @@ int mgx_crt_reconstruct(const mgx_crt_plan *pl, int64_t K, int64_t N, const uint
 
 int64_t mgx_crt_y_stride(int64_t K, int64_t N) { return crt_ystride(K * N); }
 
+int mgx_crt_plan_cache_stats(int64_t *total, int64_t *pinned)
+{
+    if (!total || !pinned) return fail(MGX_ERR_ARG, "null argument");
+    plan_cache_stats(total, pinned);
+    return MGX_OK;
+}
+
 }  // extern "C"
 
 namespace {
@@ -620,10 +632,24 @@ int mgx_crt_mfma_matmul(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t 
         const mgx_limb *X[2] = {A, B};
         int64_t bits[2];
         MGX_TRY(scan_bits_pair(2, rows, cols, X, rs, cap, bits, s));
-        MGX_TRY(cached_plan(M, bits[0], bits[1], &pl));
+        MGX_TRY(cached_plan(M, bits[0], bits[1], &pl));   // pinned until this call has synchronized
     } else if (plan->K < M) {
         return fail(MGX_ERR_SHAPE, "plan made for inner dimension %lld < %lld", (long long)plan->K, (long long)M);
     }
+    int st = crt_matmul_with(pl, K, M, N, C, Crs, A, Ars, B, Brs, cap, s);
+    if (!plan) release_plan(pl);
+    return st;
+}
+
+}  // extern "C"
+
+namespace mgx {
+// The product through a plan (stream-ordered), then one synchronization for
+// the device error word.
+int crt_matmul_with(const mgx_crt_plan *pl, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs,
+                    const mgx_limb *A, int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap, hipStream_t s)
+{
+    void *stream = s;
     int64_t rpadA, rpadB, kpad;
     mgx_crt_residue_dims(pl, K, M, &rpadA, &kpad);
     mgx_crt_residue_dims(pl, N, M, &rpadB, nullptr);
@@ -642,7 +668,8 @@ int mgx_crt_mfma_matmul(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t 
 
         if (pl->kb <= RES_KB) {   // both operands in one launch: one tail instead of two
             const int64_t nA = rpadA * kpad, nB = rpadB * kpad;
-            const ResSide sa{(const uint64_t *)A, Ars, K, nA, 0, RA}, sb{(const uint64_t *)B, Brs, N, nB, 1, RB};
+            const ResSide sa{(const uint64_t *)A, Ars, K, nA, 0, (int)pl->bitsA, RA},
+                sb{(const uint64_t *)B, Brs, N, nB, 1, (int)pl->bitsB, RB};
             auto res = pl->S == 2 ? reduce_res_for<2>(pl->kb) : reduce_res_for<4>(pl->kb);
             const int64_t nblk = ceil_div(nA, 128) + ceil_div(nB, 128);
             if (nblk > 0x7fffffffll) st = fail(MGX_ERR_SHAPE, "matrices too large for one reduce launch");
@@ -664,44 +691,88 @@ int mgx_crt_mfma_matmul(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t 
     return st;
 }
 
-}  // extern "C"
-
-namespace mgx {
 int plan_view(const mgx_crt_plan *pl, PlanView *v)
 {
     v->pc = pl->d_pc; v->cst = pl->d_cst; v->mc = pl->d_mc; v->m32 = pl->d_m32;
     v->S = pl->S; v->P = pl->P; v->kb = pl->kb; v->Kr = pl->Kr; v->rows4pad = pl->rows4pad; v->Lbp = pl->Lbp;
     v->out_limbs = pl->out_limbs;
     v->nw = pl->nw;
+    v->bitsA = pl->bitsA;
+    v->bitsB = pl->bitsB;
     return MGX_OK;
 }
 
-// Plans made for (device, ceil(log2 K), bitsA, bitsB) are kept for the life of
-// the process, so that repeated products of one shape pay for the constant
-// tables once.  At most 256 are kept; beyond that new shapes get fresh plans
-// that replace the oldest cached one.
+// Plans made for (device, ceil(log2 K), bitsA, bitsB, slice setting) are kept
+// for the life of the process, so that repeated products of one shape pay for
+// the constant tables once.  Every taker pins its plan and releases it when no
+// queued or captured work of its own reads the plan any more.  At most
+// PLAN_CACHE_MAX unpinned plans are kept: a new shape evicts the least
+// recently used unpinned plan (pinned ones are never freed, so a batch with
+// more distinct shapes than that simply keeps them all).
+namespace {
+constexpr size_t PLAN_CACHE_MAX = 256;
+struct CachedPlan {
+    std::array<int64_t, 5> key;
+    mgx_crt_plan *plan;
+    int pins;
+    uint64_t used;   // logical clock of the last take
+};
+std::mutex g_plan_mu;
+std::vector<CachedPlan> g_plans;
+uint64_t g_plan_clock = 0;
+}  // namespace
+
 int cached_plan(int64_t K, int64_t bitsA, int64_t bitsB, const mgx_crt_plan **out)
 {
-    static std::mutex mu;
-    static std::vector<std::pair<std::array<int64_t, 5>, mgx_crt_plan *>> cache;
     int dev = 0;
     MGX_HIP(hipGetDevice(&dev));
     std::array<int64_t, 5> key = {dev, clog2(K), bitsA, bitsB, param(MGX_PARAM_CRT_SLICES)};
-    std::lock_guard<std::mutex> lk(mu);
-    for (auto &kv : cache)
-        if (kv.first == key) {
-            *out = kv.second;
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    for (auto &c : g_plans)
+        if (c.key == key) {
+            c.pins++;
+            c.used = ++g_plan_clock;
+            *out = c.plan;
             return MGX_OK;
         }
     mgx_crt_plan *pl = nullptr;
     MGX_TRY(mgx_crt_plan_create(&pl, (int64_t)1 << clog2(K), bitsA, bitsB));
-    if (cache.size() >= 256) {
-        hipDeviceSynchronize();   // the evicted plan may still be in use by queued work
-        mgx_crt_plan_destroy(cache.front().second);
-        cache.erase(cache.begin());
+    size_t unpinned = 0;
+    for (auto &c : g_plans) unpinned += c.pins == 0;
+    if (unpinned >= PLAN_CACHE_MAX) {
+        size_t victim = g_plans.size();
+        for (size_t i = 0; i < g_plans.size(); i++)
+            if (g_plans[i].pins == 0 && (victim == g_plans.size() || g_plans[i].used < g_plans[victim].used))
+                victim = i;
+        // an unpinned plan has no queued work of a pinning call left (each
+        // releases after its own synchronization); the device-wide wait
+        // covers work on other streams that a caller-given plan shares
+        (void)hipDeviceSynchronize();
+        mgx_crt_plan_destroy(g_plans[victim].plan);
+        g_plans.erase(g_plans.begin() + (ptrdiff_t)victim);
     }
-    cache.push_back({key, pl});
+    g_plans.push_back({key, pl, 1, ++g_plan_clock});
     *out = pl;
     return MGX_OK;
+}
+
+void release_plan(const mgx_crt_plan *pl)
+{
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    for (auto &c : g_plans)
+        if (c.plan == pl && c.pins > 0) {
+            c.pins--;
+            return;
+        }
+}
+
+// Cache occupancy for tests: total plans and pinned plans.
+void plan_cache_stats(int64_t *total, int64_t *pinned)
+{
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    int64_t p = 0;
+    for (auto &c : g_plans) p += c.pins > 0;
+    *total = (int64_t)g_plans.size();
+    *pinned = p;
 }
 }  // namespace mgx

@@ -110,6 +110,17 @@ __device__ __forceinline__ v4i mfma16(v4i a, v4i b, v4i c)
 
 constexpr int LDS_PITCH = 80;   // 64 data bytes + 16 padding per row
 
+// True when the entry's magnitude (limbs 0 .. a-1) has a bit at or above
+// position xbits, i.e. |x| >= 2^xbits: the plan's modulus covers only
+// products of entries below its per-operand bounds.  Zero top limbs are
+// allowed; the loop usually inspects one limb (the top one).
+__device__ __forceinline__ bool over_bound(const uint64_t *__restrict__ e, int64_t a, int xbits)
+{
+    for (int64_t li = xbits >> 6; li < a; li++)
+        if (e[li] & (li == (xbits >> 6) ? ~0ull << (xbits & 63) : ~0ull)) return true;
+    return false;
+}
+
 
 // Byte offset of residue (row r, column k) in an S = 2 residue plane: blocks
 // of 16 rows x 64 columns (1 KB, row-major inside), the blocks of a 16-row band
@@ -157,7 +168,7 @@ __device__ __forceinline__ void store_res_tile(const uint32_t *LO4, int tid, int
 // ------------------------------------------------------------------------
 template <int S>
 __device__ __forceinline__ void reduce_tile(int64_t blk,
-    const PrimeConst *__restrict__ pc, const int8_t *__restrict__ cst, int P, int nrb, int kb,
+    const PrimeConst *__restrict__ pc, const int8_t *__restrict__ cst, int P, int nrb, int kb, int xbits,
     const uint64_t *__restrict__ X, int64_t Xrs, int64_t W, int transpose, int64_t rows, int64_t K,
     int64_t kpad, int64_t nent, int8_t *__restrict__ R, int64_t plane, unsigned *__restrict__ err)
 {
@@ -179,9 +190,7 @@ __device__ __forceinline__ void reduce_tile(int64_t blk,
         const int64_t sz = (int64_t)X[base[h] + W - 1];
         const int64_t a = real ? min(sz < 0 ? -sz : sz, W - 1) : 0;
         nl[h] = (int)min<int64_t>(a, lim);
-        if (rb == 0)   // limbs beyond the plan's bit bound must be zero
-            for (int64_t li = kb / 8; li < a; li++)
-                if (X[base[h] + li]) { atomicOr(err, DEV_ERR_INPUT); break; }
+        if (rb == 0 && over_bound(X + base[h], a, xbits)) atomicOr(err, DEV_ERR_INPUT);
     }
     const int part = tid & 3;
     v16i acc[2];
@@ -293,7 +302,7 @@ constexpr int RES_KG = 2;   // K steps whose A fragments are read ahead of their
 
 template <int S, int KS>
 __device__ __forceinline__ void reduce_tile_res(int64_t blk,
-    const PrimeConst *__restrict__ pc, const int8_t *__restrict__ cst, int P, int nrb, int kb,
+    const PrimeConst *__restrict__ pc, const int8_t *__restrict__ cst, int P, int nrb, int kb, int xbits,
     const uint64_t *__restrict__ X, int64_t Xrs, int64_t W, int transpose, int64_t rows, int64_t K,
     int64_t kpad, int8_t *__restrict__ R, int64_t plane, unsigned *__restrict__ err)
 {
@@ -340,9 +349,7 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
             const uint64_t d1 = (4 * k32 + 2 * hh + 1 < nl ? v[k32][1] : 0) ^ flip;
             bf[k32] = (v4i){(int)(uint32_t)d0, (int)(uint32_t)(d0 >> 32), (int)(uint32_t)d1, (int)(uint32_t)(d1 >> 32)};
         }
-        if (hh == 0)
-            for (int64_t x = kb / 8; x < a; x++)   // limbs beyond the plan's bound must be zero
-                if (X[base + x]) { atomicOr(err, DEV_ERR_INPUT); break; }
+        if (hh == 0 && over_bound(X + base, a, xbits)) atomicOr(err, DEV_ERR_INPUT);
     }
     // the constants of row block rb: 64 rows x kb bytes, at most KS 16-byte
     // chunks per thread, one block ahead in registers
@@ -1191,8 +1198,18 @@ struct PlanView {
     const int8_t *cst, *mc;
     const uint32_t *m32;
     int64_t S, P, kb, Kr, rows4pad, Lbp, out_limbs, nw;
+    int64_t bitsA, bitsB;   // the operands' bit bounds (|A| < 2^bitsA, |B| < 2^bitsB)
 };
 int plan_view(const mgx_crt_plan *pl, PlanView *v);
+// A plan from the per-device cache, pinned: it is not evicted (freed) until
+// every pin is released with release_plan.  The batched executor holds its
+// plans pinned for its lifetime, a one-product call for the call.
 int cached_plan(int64_t K, int64_t bitsA, int64_t bitsB, const mgx_crt_plan **out);
+void release_plan(const mgx_crt_plan *pl);
+void plan_cache_stats(int64_t *total, int64_t *pinned);
+// C = A*B through a plan: stage launches on s, then one synchronization for
+// the device error word.
+int crt_matmul_with(const mgx_crt_plan *pl, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs,
+                    const mgx_limb *A, int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap, hipStream_t s);
 
 }  // namespace mgx

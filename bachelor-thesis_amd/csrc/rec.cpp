@@ -235,6 +235,17 @@ int squarify_nosync(int method, int64_t K, int64_t M, int64_t N, uint64_t *C, in
     Ctx c = make_ctx(s, cap, false);
     return squarify(c, method, K, M, N, C, Crs, A, Ars, B, Brs, opbits);
 }
+// the same with the leaf size and depth given per call (the routes and the
+// tuner; the global tunables are left alone, so concurrent calls do not race)
+int squarify_with(int method, int64_t K, int64_t M, int64_t N, uint64_t *C, int64_t Crs, const uint64_t *A,
+                  int64_t Ars, const uint64_t *B, int64_t Brs, int64_t cap, int64_t opbits, int64_t leaf,
+                  int64_t maxdepth, hipStream_t s)
+{
+    Ctx c = make_ctx(s, cap, false);
+    c.leaf = std::max<int64_t>(leaf, 1);
+    c.maxdepth = maxdepth;
+    return squarify(c, method, K, M, N, C, Crs, A, Ars, B, Brs, opbits);
+}
 }  // namespace mgx
 
 extern "C" {

@@ -12,8 +12,9 @@
 namespace mgx {
 int plain_mul(bool strided, bool addmul, int64_t K, int64_t M, int64_t N, uint64_t *C, int64_t Crs, const uint64_t *A,
               int64_t Ars, const uint64_t *B, int64_t Brs, int64_t cap, int64_t opwords, hipStream_t s);
-int squarify_nosync(int method, int64_t K, int64_t M, int64_t N, uint64_t *C, int64_t Crs, const uint64_t *A,
-                    int64_t Ars, const uint64_t *B, int64_t Brs, int64_t cap, int64_t opbits, hipStream_t s);
+int squarify_with(int method, int64_t K, int64_t M, int64_t N, uint64_t *C, int64_t Crs, const uint64_t *A,
+                  int64_t Ars, const uint64_t *B, int64_t Brs, int64_t cap, int64_t opbits, int64_t leaf,
+                  int64_t maxdepth, hipStream_t s);
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z)
 {
@@ -88,13 +89,8 @@ int plain_route(int route, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t
                 const mgx_limb *B, int64_t Brs, int64_t cap, int64_t opbits, hipStream_t s)
 {
     if (route == MGX_ROUTE_PLAIN) return plain_mul(false, false, K, M, N, C, Crs, A, Ars, B, Brs, cap, (opbits + 31) / 32, s);
-    int64_t oldleaf = mgx_get_param(MGX_PARAM_LEAF), olddepth = mgx_get_param(MGX_PARAM_WINOGRAD_DEPTH);
-    mgx_set_param(MGX_PARAM_LEAF, 2);
-    mgx_set_param(MGX_PARAM_WINOGRAD_DEPTH, route == MGX_ROUTE_WINOGRAD1 ? 1 : 2);
-    int st = squarify_nosync(MGX_REC_WINOGRAD, K, M, N, C, Crs, A, Ars, B, Brs, cap, opbits, s);
-    mgx_set_param(MGX_PARAM_LEAF, oldleaf);
-    mgx_set_param(MGX_PARAM_WINOGRAD_DEPTH, olddepth);
-    return st;
+    return squarify_with(MGX_REC_WINOGRAD, K, M, N, C, Crs, A, Ars, B, Brs, cap, opbits, 2,
+                         route == MGX_ROUTE_WINOGRAD1 ? 1 : 2, s);
 }
 
 struct Stats {
@@ -261,9 +257,15 @@ int mgx_dispatch_choose(int64_t K, int64_t M, int64_t N, const mgx_limb *A, int6
     if (cm) {
         float y[MGX_ROUTES];
         MGX_TRY(mgx_costmodel_predict(cm, f, 1, y));
-        int best = 0;
-        for (int r = 1; r < MGX_ROUTES; r++)
-            if (y[r] < y[best]) best = r;
+        // the plain kernel takes operands of at most 512 32-bit words, and
+        // each Winograd level adds up to two bits to them: routes that
+        // cannot take the product are never chosen, whatever the prediction
+        const double maxbits = std::exp2((double)f[5]);
+        int best = MGX_ROUTE_CRT;
+        for (int r = 1; r < MGX_ROUTES; r++) {
+            const int levels = r == MGX_ROUTE_WINOGRAD1 ? 1 : r == MGX_ROUTE_WINOGRAD2 ? 2 : 0;
+            if (maxbits + 2 * levels <= 512 * 32 && y[r] < y[best]) best = r;
+        }
         *route = best;
         return MGX_OK;
     }

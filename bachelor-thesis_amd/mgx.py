@@ -82,6 +82,7 @@ def _declare(L):
         "mgx_crt_plan_create": [ctypes.POINTER(c_vp), c_i64, c_i64, c_i64],
         "mgx_crt_plan_destroy": [c_vp],
         "mgx_crt_plan_prime_list": [c_vp, c_vp],
+        "mgx_crt_plan_cache_stats": [ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)],
         "mgx_crt_scan_bits": [c_i64, c_i64, c_vp, c_i64, c_i64, ctypes.POINTER(c_i64), c_vp],
         "mgx_crt_mfma_matmul": [c_vp] + mul_args,
         "mgx_crt_residue_dims": [c_vp, c_i64, c_i64, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)],
@@ -90,13 +91,49 @@ def _declare(L):
         "mgx_crt_reconstruct": [c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_i64, c_vp],
         "mgx_set_param": [ctypes.c_int, c_i64],
         "mgx_get_param": [ctypes.c_int],
+        # kept reference CRT API (host)
+        "mgx_crt_split": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64],
+        "mgx_crt_merge": [c_vp] * 7 + [c_i64],
+        "mgx_crt_make_split_tree": [ctypes.POINTER(c_vp)] * 4 + [c_vp, ctypes.c_uint64, c_i64],
+        "mgx_crt_split_list": [c_vp, c_vp, c_vp, c_vp, ctypes.c_int, c_i64],
+        "mgx_crt_merge_list": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_int, c_i64],
+        # mgx_sched.h
+        "mgx_matmul_route": [ctypes.c_int] + mul_args,
+        "mgx_random_matrix_device": [c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_i64, ctypes.c_double,
+                                     ctypes.c_uint64, c_vp],
+        "mgx_tune_features": [c_i64, c_i64, c_i64, c_vp, ctypes.c_double, c_i64, ctypes.c_double, c_vp],
+        "mgx_tune_features_device": [c_i64, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_i64, c_vp],
+        "mgx_tune_time_routes": [c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, ctypes.c_int, c_vp,
+                                 ctypes.POINTER(ctypes.c_uint)],
+        "mgx_costmodel_create": [ctypes.POINTER(c_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64],
+        "mgx_costmodel_destroy": [c_vp],
+        "mgx_costmodel_get_params": [c_vp, c_vp],
+        "mgx_costmodel_set_params": [c_vp, c_vp],
+        "mgx_costmodel_set_normalisation": [c_vp, c_vp, c_vp],
+        "mgx_costmodel_loss_grad": [c_vp, c_vp, c_vp, c_vp, ctypes.c_int, c_vp, ctypes.POINTER(ctypes.c_float)],
+        "mgx_costmodel_train": [c_vp, c_vp, c_vp, c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                ctypes.POINTER(ctypes.c_float)],
+        "mgx_costmodel_predict": [c_vp, c_vp, ctypes.c_int, c_vp],
+        "mgx_costmodel_load": [ctypes.POINTER(c_vp), ctypes.c_char_p],
+        "mgx_dispatch_set_model": [c_vp],
+        "mgx_dispatch_choose": [c_i64, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_i64, ctypes.POINTER(ctypes.c_int)],
+        "mgx_batch_create": [ctypes.POINTER(c_vp), c_vp, c_i64, c_vp],
+        "mgx_batch_run": [c_vp, c_vp],
+        "mgx_batch_route": [c_vp, c_i64],
+        "mgx_batch_destroy": [c_vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
         f.argtypes = args
         f.restype = ctypes.c_int
+    L.mgx_last_error.argtypes = []
     L.mgx_last_error.restype = ctypes.c_char_p
+    L.mgx_version.argtypes = []
     L.mgx_version.restype = ctypes.c_char_p
+    L.mgx_costmodel_param_count.argtypes = [c_vp]
+    L.mgx_costmodel_param_count.restype = c_i64
+    L.mgx_batch_scratch_bytes.argtypes = [c_vp]
+    L.mgx_batch_scratch_bytes.restype = c_i64
     L.mgx_entry_to_hex.restype = c_i64
     L.mgx_get_param.restype = c_i64
     L.mgx_crt_plan_primes.argtypes = [c_vp]
@@ -115,22 +152,9 @@ class Product(ctypes.Structure):
 
 
 def bind_sched():
-    """Argument types of the batch, dispatch and device-generator entry points."""
-    L = lib()
-    L.mgx_batch_create.argtypes = [ctypes.POINTER(c_vp), c_vp, c_i64, c_vp]
-    L.mgx_batch_run.argtypes = [c_vp, c_vp]
-    L.mgx_batch_route.argtypes = [c_vp, c_i64]
-    L.mgx_batch_scratch_bytes.argtypes = [c_vp]
-    L.mgx_batch_scratch_bytes.restype = c_i64
-    L.mgx_batch_destroy.argtypes = [c_vp]
-    L.mgx_random_matrix_device.argtypes = [c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_i64, ctypes.c_double,
-                                           ctypes.c_uint64, c_vp]
-    L.mgx_matmul_route.argtypes = [ctypes.c_int, c_i64, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i64,
-                                   c_vp]
-    L.mgx_costmodel_load.argtypes = [ctypes.POINTER(c_vp), ctypes.c_char_p]
-    L.mgx_costmodel_destroy.argtypes = [c_vp]
-    L.mgx_dispatch_set_model.argtypes = [c_vp]
-    return L
+    """The library with every entry point of mgx_sched.h declared (all of
+    them are declared when the library is first loaded)."""
+    return lib()
 
 
 def check(status):

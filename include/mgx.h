@@ -194,7 +194,9 @@ int64_t mgx_crt_plan_primes(const mgx_crt_plan *plan);
 int64_t mgx_crt_plan_slices(const mgx_crt_plan *plan);   /* S: 2 (primes < 65280) or 4 (primes < 2^31) */
 int mgx_crt_plan_prime_list(const mgx_crt_plan *plan, uint32_t *out);
 /* C = A*B through the plan (plan NULL: scan the inputs' bit lengths on the
- * device, which synchronizes the stream once, and plan internally). */
+ * device, which synchronizes the stream once, and plan internally).  With a
+ * caller's plan, M must not exceed its K and every |A| < 2^bitsA, |B| < 2^bitsB
+ * (checked on the device: MGX_ERR_SHAPE otherwise, C undefined). */
 int mgx_crt_mfma_matmul(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs,
                         const mgx_limb *A, int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap,
                         void *stream);
@@ -210,6 +212,10 @@ int mgx_crt_mfma_matmul(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t 
  * mgx_crt_reconstruct takes its own stream-ordered scratch (about
  * 8 * ystride bytes per 32-bit word of M). */
 int mgx_crt_residue_dims(const mgx_crt_plan *plan, int64_t rows, int64_t K, int64_t *rpad, int64_t *kpad);
+/* mgx_crt_reduce: transpose 0 reduces the K-major operand A (rows x K, bound
+ * bitsA), transpose 1 the operand B stored K x rows (bound bitsB); an entry at
+ * or above its operand's bound sets the device error word (MGX_ERR_SHAPE at
+ * the next check).  nbytes is reserved and must be 0. */
 int mgx_crt_reduce(const mgx_crt_plan *plan, int transpose, int64_t rows, int64_t K, const mgx_limb *X,
                    int64_t Xrs, int64_t cap, int64_t nbytes, int8_t *R, void *stream);
 int mgx_crt_residue_gemm(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t N, const int8_t *RA,
@@ -217,6 +223,11 @@ int mgx_crt_residue_gemm(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t
 int mgx_crt_reconstruct(const mgx_crt_plan *plan, int64_t K, int64_t N, const uint32_t *Y, mgx_limb *C,
                         int64_t Crs, int64_t cap, void *stream);
 int64_t mgx_crt_y_stride(int64_t K, int64_t N);
+/* The library's per-device plan cache (plans made for mgx_crt_mfma_matmul
+ * with plan NULL and for batches): number of cached plans and how many are
+ * pinned by a call or a live batch.  Pinned plans are never evicted; at most
+ * 256 unpinned ones are kept.  Housekeeping with no reference counterpart. */
+int mgx_crt_plan_cache_stats(int64_t *total, int64_t *pinned);
 
 /* ---------------- kept reference CRT API (mat-gpu/gpu_crt.h:6-14; host) ---------------- */
 /* Same arguments and meaning as gpu_crt_* on entries of capacity cap, using

@@ -78,3 +78,33 @@ def rand_ints(rows, cols, bits, seed, mode="rand"):
 def matmul(A, B):
     K, M, N = len(A), len(B), len(B[0]) if B else 0
     return [[sum(A[y][k] * B[k][x] for k in range(M)) for x in range(N)] for y in range(K)]
+
+
+def mod_q_entries(X, q, chunk=65536):
+    """Every entry of an entry matrix mod q (signed entries mapped to [0, q)),
+    for q < 2^26 and at most 2048 limbs per entry: limb i contributes
+    (limb mod q) (2^(64 i) mod q) < 2^52, so a row of them sums below 2^63."""
+    cap = X.shape[-1] - 1
+    flat = X.reshape(-1, cap + 1)
+    sz = flat[:, cap].view(np.int64)
+    n = np.abs(sz)
+    L = int(n.max()) if n.size else 0
+    out = np.zeros(flat.shape[0], dtype=np.uint64)
+    if L:
+        assert L <= 2048 and q < (1 << 26)
+        pw = np.array([pow(2, 64 * i, q) for i in range(L)], dtype=np.uint64)
+        qq = np.uint64(q)
+        for c0 in range(0, flat.shape[0], chunk):
+            lim = flat[c0:c0 + chunk, :L] % qq
+            lim[np.arange(L)[None, :] >= n[c0:c0 + chunk, None]] = 0
+            out[c0:c0 + chunk] = (lim * pw[None, :]).sum(axis=1) % qq
+    out = np.where(sz < 0, (np.uint64(q) - out) % np.uint64(q), out)
+    return out.astype(np.int64).reshape(X.shape[:-1])
+
+
+def matmul_mod(a, b, q):
+    """(a @ b) mod q for int64 matrices with entries in [0, q), q < 2^26."""
+    out = np.zeros((a.shape[0], b.shape[1]), dtype=np.int64)
+    for k0 in range(0, a.shape[1], 256):   # partial sums stay below 2^63
+        out = (out + a[:, k0:k0 + 256] @ b[k0:k0 + 256, :]) % q
+    return out

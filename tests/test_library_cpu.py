@@ -30,6 +30,16 @@ def test_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_binding_declares_every_symbol():
+    """Every function the headers declare has its argument types set in the
+    Python binding: ctypes would otherwise pass 64-bit pointers and sizes as
+    32-bit C ints, truncating device addresses."""
+    import mgx
+    L = mgx.lib()
+    undeclared = [n for n in sorted(declared_symbols()) if getattr(L, n).argtypes is None]
+    assert not undeclared, undeclared
+
+
 def _xoshiro_entries(rows, cols, cap, nbits, seed):
     M64 = (1 << 64) - 1
 
