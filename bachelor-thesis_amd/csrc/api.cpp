@@ -35,11 +35,14 @@ int fail(int status, const char *fmt, ...)
     return status;
 }
 
+int noalloc_scratch_init(int dev);   // crt.hip
+
 int ensure_init()
 {
     std::lock_guard<std::mutex> lk(g_mu);
     int dev = 0;
     MGX_HIP(hipGetDevice(&dev));
+    MGX_TRY(noalloc_scratch_init(dev));
     if ((int)g_errword.size() <= dev) g_errword.resize(dev + 1, nullptr);
     if (!g_errword[dev]) {
         unsigned *w = nullptr;

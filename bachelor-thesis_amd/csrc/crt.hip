@@ -644,51 +644,110 @@ int mgx_crt_mfma_matmul(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t 
 }  // extern "C"
 
 namespace mgx {
+namespace {
+// One scratch block per product: residues of A and B, Y, then the
+// reconstruction's column sums and gamma (which reuse the residue bytes when
+// those are larger).  Every byte a stage reads was written by an earlier
+// stage of the same product (the residue planes including their zero
+// padding), so the block needs no clearing.
+struct CrtLayout {
+    int64_t rpadA, rpadB, kpad;
+    size_t bytesA, offY, offR, total;
+};
+CrtLayout crt_layout(const mgx_crt_plan *pl, int64_t K, int64_t M, int64_t N)
+{
+    CrtLayout L;
+    mgx_crt_residue_dims(pl, K, M, &L.rpadA, &L.kpad);
+    mgx_crt_residue_dims(pl, N, M, &L.rpadB, nullptr);
+    L.bytesA = (size_t)(pl->S * pl->P * L.rpadA * L.kpad);
+    const size_t bytesB = (size_t)(pl->S * pl->P * L.rpadB * L.kpad);
+    const size_t bytesY = (size_t)ceil_div(pl->S * pl->P, 4) * crt_ystride(K * N) * 4;
+    const size_t bytesR = recon_scratch_bytes(pl, K * N);
+    L.offY = (size_t)round_up((int64_t)(L.bytesA + bytesB), 256);
+    L.offR = bytesR <= L.offY ? 0 : (size_t)round_up((int64_t)(L.offY + bytesY), 256);
+    L.total = std::max(L.offY + bytesY, L.offR + bytesR);
+    return L;
+}
+
+// The stage launches of one product on s (no allocation, no synchronization).
+int crt_launch(const mgx_crt_plan *pl, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A,
+               int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap, char *scr, const CrtLayout &L, hipStream_t s)
+{
+    void *stream = s;
+    int8_t *RA = (int8_t *)scr, *RB = (int8_t *)scr + L.bytesA;
+    uint32_t *Y = (uint32_t *)(scr + L.offY);
+    if (pl->kb <= RES_KB) {   // both operands in one launch: one tail instead of two
+        const int64_t nA = L.rpadA * L.kpad, nB = L.rpadB * L.kpad;
+        const ResSide sa{(const uint64_t *)A, Ars, K, nA, 0, (int)pl->bitsA, RA},
+            sb{(const uint64_t *)B, Brs, N, nB, 1, (int)pl->bitsB, RB};
+        auto res = pl->S == 2 ? reduce_res_for<2>(pl->kb) : reduce_res_for<4>(pl->kb);
+        const int64_t nblk = ceil_div(nA, 128) + ceil_div(nB, 128);
+        if (nblk > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "matrices too large for one reduce launch");
+        hipLaunchKernelGGL(res, dim3((unsigned)nblk), dim3(256), 0, s, pl->d_pc, pl->d_cst, (int)pl->P,
+                           (int)(pl->rows4pad / 64), (int)pl->kb, cap + 1, M, L.kpad, sa, sb, ceil_div(nA, 128),
+                           device_error_word());
+        MGX_HIP(hipGetLastError());
+    } else {
+        MGX_TRY(mgx_crt_reduce(pl, 0, K, M, A, Ars, cap, 0, RA, stream));
+        MGX_TRY(mgx_crt_reduce(pl, 1, N, M, B, Brs, cap, 0, RB, stream));
+    }
+    MGX_TRY(mgx_crt_residue_gemm(pl, K, M, N, RA, RB, Y, stream));
+    return reconstruct_with(pl, K, N, Y, C, Crs, cap, scr + L.offR, s);
+}
+}  // namespace
+
 // The product through a plan (stream-ordered), then one synchronization for
 // the device error word.
 int crt_matmul_with(const mgx_crt_plan *pl, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs,
                     const mgx_limb *A, int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap, hipStream_t s)
 {
-    void *stream = s;
-    int64_t rpadA, rpadB, kpad;
-    mgx_crt_residue_dims(pl, K, M, &rpadA, &kpad);
-    mgx_crt_residue_dims(pl, N, M, &rpadB, nullptr);
-    // one scratch block: residues of A and B, Y, then the reconstruction's
-    // column sums and gamma (which reuse the residue bytes when those are larger)
-    const size_t bytesA = (size_t)(pl->S * pl->P * rpadA * kpad), bytesB = (size_t)(pl->S * pl->P * rpadB * kpad);
-    const size_t bytesY = (size_t)ceil_div(pl->S * pl->P, 4) * crt_ystride(K * N) * 4;
-    const size_t bytesR = recon_scratch_bytes(pl, K * N);
-    const size_t offY = (size_t)round_up((int64_t)(bytesA + bytesB), 256);
-    const size_t offR = bytesR <= offY ? 0 : (size_t)round_up((int64_t)(offY + bytesY), 256);
+    const CrtLayout L = crt_layout(pl, K, M, N);
     char *scr = nullptr;
-    int st = scratch_alloc((void **)&scr, std::max(offY + bytesY, offR + bytesR), s);
+    int st = scratch_alloc((void **)&scr, L.total, s);
     if (st == MGX_OK) {
-        int8_t *RA = (int8_t *)scr, *RB = (int8_t *)scr + bytesA;
-        uint32_t *Y = (uint32_t *)(scr + offY);
-
-        if (pl->kb <= RES_KB) {   // both operands in one launch: one tail instead of two
-            const int64_t nA = rpadA * kpad, nB = rpadB * kpad;
-            const ResSide sa{(const uint64_t *)A, Ars, K, nA, 0, (int)pl->bitsA, RA},
-                sb{(const uint64_t *)B, Brs, N, nB, 1, (int)pl->bitsB, RB};
-            auto res = pl->S == 2 ? reduce_res_for<2>(pl->kb) : reduce_res_for<4>(pl->kb);
-            const int64_t nblk = ceil_div(nA, 128) + ceil_div(nB, 128);
-            if (nblk > 0x7fffffffll) st = fail(MGX_ERR_SHAPE, "matrices too large for one reduce launch");
-            else {
-                hipLaunchKernelGGL(res, dim3((unsigned)nblk), dim3(256), 0, s, pl->d_pc, pl->d_cst, (int)pl->P,
-                                   (int)(pl->rows4pad / 64), (int)pl->kb, cap + 1, M, kpad, sa, sb,
-                                   ceil_div(nA, 128), device_error_word());
-                if (hipGetLastError() != hipSuccess) st = fail(MGX_ERR_HIP, "reduce launch");
-            }
-        } else {
-            st = mgx_crt_reduce(pl, 0, K, M, A, Ars, cap, 0, RA, stream);
-            if (st == MGX_OK) st = mgx_crt_reduce(pl, 1, N, M, B, Brs, cap, 0, RB, stream);
-        }
-        if (st == MGX_OK) st = mgx_crt_residue_gemm(pl, K, M, N, RA, RB, Y, stream);
-        if (st == MGX_OK) st = reconstruct_with(pl, K, N, Y, C, Crs, cap, scr + offR, s);
+        st = crt_launch(pl, K, M, N, C, Crs, A, Ars, B, Brs, cap, scr, L, s);
         scratch_free(scr, s);
     }
     if (st == MGX_OK) st = check_device_error(s);
     return st;
+}
+
+// ------------------------------------------------------------------------
+// The entry point that allocates nothing: its scratch block is one
+// file-scope device array per GPU, of NOALLOC_SCRATCH_BYTES, held until the
+// process ends.  In host code `scratch` names a host-side stand-in, never
+// the device memory: its device address is looked up once per device (at
+// library init, with that device current) and only that address is passed
+// to kernels.
+// ------------------------------------------------------------------------
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)MGX_NOALLOC_SCRATCH_MB << 20;
+static __device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+char *g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+int noalloc_scratch_init(int dev)
+{
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(MGX_ERR_UNSUPPORTED, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[dev]) return MGX_OK;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(MGX_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_noalloc_ptr[dev] = (char *)p;
+    return MGX_OK;
+}
+
+__global__ void zero_entries_kernel(uint64_t *C, int64_t Crs, int64_t K, int64_t N, int64_t W)
+{
+    const int64_t total = K * N * W;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = t / W;
+        C[((e / N) * Crs + e % N) * W + t % W] = 0;
+    }
 }
 
 int plan_view(const mgx_crt_plan *pl, PlanView *v)
@@ -776,3 +835,46 @@ void plan_cache_stats(int64_t *total, int64_t *pinned)
     *pinned = p;
 }
 }  // namespace mgx
+
+extern "C" {
+
+int mgx_crt_mfma_matmul_noalloc(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs,
+                                const mgx_limb *A, int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap,
+                                void *stream)
+{
+    if (!plan) return fail(MGX_ERR_ARG, "the no-allocation product needs a plan made with mgx_crt_plan_create");
+    if (K < 0 || M < 0 || N < 0 || cap < 1) return fail(MGX_ERR_SHAPE, "bad shape");
+    if (plan->K < M)
+        return fail(MGX_ERR_SHAPE, "plan made for inner dimension %lld < %lld", (long long)plan->K, (long long)M);
+    int dev = 0;
+    MGX_HIP(hipGetDevice(&dev));
+    char *scr = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_noalloc_mu);
+        if (dev >= 0 && dev < NOALLOC_MAX_DEVICES) scr = g_noalloc_ptr[dev];
+    }
+    if (!scr || !device_error_word())
+        return fail(MGX_ERR_ARG, "call mgx_init on device %d before the no-allocation product", dev);
+    if (K == 0 || N == 0) return MGX_OK;
+    hipStream_t s = as_stream(stream);
+    if (M == 0) {   // empty inner dimension: C = 0 (a fill kernel, not a memset)
+        const int64_t total = K * N * (cap + 1);
+        hipLaunchKernelGGL(zero_entries_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(total, 256), 4096)),
+                           dim3(256), 0, s, (uint64_t *)C, Crs, K, N, cap + 1);
+        MGX_HIP(hipGetLastError());
+        return MGX_OK;
+    }
+    const CrtLayout L = crt_layout(plan, K, M, N);
+    if (L.total > NOALLOC_SCRATCH_BYTES)
+        return fail(MGX_ERR_SHAPE, "this product needs %zu bytes of scratch; the no-allocation entry point has %zu",
+                    L.total, NOALLOC_SCRATCH_BYTES);
+    return crt_launch(plan, K, M, N, C, Crs, A, Ars, B, Brs, cap, scr, L, s);
+}
+
+int64_t mgx_crt_noalloc_scratch_bytes(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t N)
+{
+    if (!plan || K < 0 || M < 1 || N < 0) return -1;
+    return (int64_t)crt_layout(plan, K, M, N).total;
+}
+
+}  // extern "C"

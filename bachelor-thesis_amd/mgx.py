@@ -85,6 +85,7 @@ def _declare(L):
         "mgx_crt_plan_cache_stats": [ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)],
         "mgx_crt_scan_bits": [c_i64, c_i64, c_vp, c_i64, c_i64, ctypes.POINTER(c_i64), c_vp],
         "mgx_crt_mfma_matmul": [c_vp] + mul_args,
+        "mgx_crt_mfma_matmul_noalloc": [c_vp] + mul_args,
         "mgx_crt_residue_dims": [c_vp, c_i64, c_i64, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)],
         "mgx_crt_reduce": [c_vp, ctypes.c_int, c_i64, c_i64, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp],
         "mgx_crt_residue_gemm": [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp],
@@ -140,6 +141,8 @@ def _declare(L):
     L.mgx_crt_plan_primes.restype = c_i64
     L.mgx_crt_y_stride.argtypes = [c_i64, c_i64]
     L.mgx_crt_y_stride.restype = c_i64
+    L.mgx_crt_noalloc_scratch_bytes.argtypes = [c_vp, c_i64, c_i64, c_i64]
+    L.mgx_crt_noalloc_scratch_bytes.restype = c_i64
     L.mgx_crt_plan_slices.argtypes = [c_vp]
     L.mgx_crt_plan_slices.restype = c_i64
 

@@ -200,6 +200,24 @@ int mgx_crt_plan_prime_list(const mgx_crt_plan *plan, uint32_t *out);
 int mgx_crt_mfma_matmul(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs,
                         const mgx_limb *A, int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap,
                         void *stream);
+/* The same product with no allocation of any kind, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Its scratch is one
+ * fixed device array per GPU of MGX_NOALLOC_SCRATCH_MB MiB, held from
+ * library load until the process ends; products needing more scratch than
+ * that (mgx_crt_noalloc_scratch_bytes) fail with MGX_ERR_SHAPE.  plan must be
+ * a caller's plan (mgx_crt_plan_create) and mgx_init must have run on the
+ * device.  Fully asynchronous: it only enqueues kernels on the stream and
+ * does not synchronize; read errors with mgx_check_error.  Every call on one
+ * GPU uses that GPU's one scratch array: calls that overlap on a GPU (other
+ * streams, other threads, an earlier call still running) race and corrupt
+ * each other's results; order them on one stream. */
+#define MGX_NOALLOC_SCRATCH_MB 128
+int mgx_crt_mfma_matmul_noalloc(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs,
+                                const mgx_limb *A, int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap,
+                                void *stream);
+/* Scratch bytes a product needs on the CRT route (-1 on bad arguments). */
+int64_t mgx_crt_noalloc_scratch_bytes(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t N);
 /* Stage kernels, exposed for testing and profiling.  Residues are planar
  * int8 slices R[q][s] (s < S) of rpad x kpad bytes (mgx_crt_residue_dims):
  * row-major for S = 4; for S = 2 in blocks of 16 rows x 64 columns (1 KB,
