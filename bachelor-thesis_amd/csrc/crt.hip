@@ -49,6 +49,10 @@ struct ScanArgs {
     int nblk0;
 };
 
+// EXACT: bit length of the top non-zero limb (two cache lines per entry:
+// size word and top limb); otherwise the limb-granular bound 64 |size| from
+// the size words alone (one cache line per entry), which is what plans need.
+template <bool EXACT>
 __global__ __launch_bounds__(256) void scan_bits_kernel(ScanArgs sa, int64_t W, int *out_pair)
 {
     __shared__ int s_best[4];
@@ -70,15 +74,20 @@ __global__ __launch_bounds__(256) void scan_bits_kernel(ScanArgs sa, int64_t W, 
             int64_t s = e < n ? (int64_t)p[j][W - 1] : 0;
             a[j] = min(s < 0 ? -s : s, W - 1);
         }
+        if (EXACT) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) top[j] = a[j] > 0 ? p[j][a[j] - 1] : 0;
+            for (int j = 0; j < 4; j++) top[j] = a[j] > 0 ? p[j][a[j] - 1] : 0;
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            while (a[j] > 0 && top[j] == 0) {   // zero top limbs: not normalized
-                a[j]--;
-                top[j] = a[j] > 0 ? p[j][a[j] - 1] : 0;
+            for (int j = 0; j < 4; j++) {
+                while (a[j] > 0 && top[j] == 0) {   // zero top limbs: not normalized
+                    a[j]--;
+                    top[j] = a[j] > 0 ? p[j][a[j] - 1] : 0;
+                }
+                if (a[j] > 0) best = max(best, (int)((a[j] - 1) * 64 + 64 - __clzll(top[j])));
             }
-            if (a[j] > 0) best = max(best, (int)((a[j] - 1) * 64 + 64 - __clzll(top[j])));
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++) best = max(best, (int)(64 * a[j]));
         }
     }
     for (int o = 32; o > 0; o >>= 1) best = max(best, __shfl_xor(best, o));
@@ -578,7 +587,7 @@ int mgx_crt_plan_cache_stats(int64_t *total, int64_t *pinned)
 namespace {
 // Bit lengths of up to two matrices with one stream synchronization.
 int scan_bits_pair(int nmat, const int64_t *rows, const int64_t *cols, const mgx_limb *const *X, const int64_t *Xrs,
-                   int64_t cap, int64_t *bits, hipStream_t s)
+                   int64_t cap, int64_t *bits, hipStream_t s, bool exact)
 {
     int *d = nullptr;
     MGX_TRY(scratch_alloc((void **)&d, 2 * sizeof(int), s));
@@ -595,7 +604,8 @@ int scan_bits_pair(int nmat, const int64_t *rows, const int64_t *cols, const mgx
     }
     sa.nblk0 = grid[0];
     if (grid[0] + grid[1] > 0)
-        hipLaunchKernelGGL(scan_bits_kernel, dim3(grid[0] + grid[1]), dim3(256), 0, s, sa, cap + 1, d);
+        hipLaunchKernelGGL(exact ? scan_bits_kernel<true> : scan_bits_kernel<false>, dim3(grid[0] + grid[1]), dim3(256),
+                           0, s, sa, cap + 1, d);
     int h[2] = {0, 0};
     MGX_HIP(hipMemcpyAsync(h, d, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
     MGX_TRY(scratch_free(d, s));
@@ -611,7 +621,7 @@ int mgx_crt_scan_bits(int64_t rows, int64_t cols, const mgx_limb *X, int64_t Xrs
                       void *stream)
 {
     MGX_TRY(ensure_init());
-    return scan_bits_pair(1, &rows, &cols, &X, &Xrs, cap, bits, as_stream(stream));
+    return scan_bits_pair(1, &rows, &cols, &X, &Xrs, cap, bits, as_stream(stream), true);
 }
 
 int mgx_crt_mfma_matmul(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs,
@@ -631,7 +641,9 @@ int mgx_crt_mfma_matmul(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t 
         const int64_t rows[2] = {K, M}, cols[2] = {M, N}, rs[2] = {Ars, Brs};
         const mgx_limb *X[2] = {A, B};
         int64_t bits[2];
-        MGX_TRY(scan_bits_pair(2, rows, cols, X, rs, cap, bits, s));
+        // limb-granular bounds (64 |size|): one cache line per entry, and the
+        // same plan as exact bit lengths whenever the top limbs are full
+        MGX_TRY(scan_bits_pair(2, rows, cols, X, rs, cap, bits, s, false));
         MGX_TRY(cached_plan(M, bits[0], bits[1], &pl));   // pinned until this call has synchronized
     } else if (plan->K < M) {
         return fail(MGX_ERR_SHAPE, "plan made for inner dimension %lld < %lld", (long long)plan->K, (long long)M);

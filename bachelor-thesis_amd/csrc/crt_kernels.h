@@ -1201,15 +1201,4 @@ struct PlanView {
     int64_t bitsA, bitsB;   // the operands' bit bounds (|A| < 2^bitsA, |B| < 2^bitsB)
 };
 int plan_view(const mgx_crt_plan *pl, PlanView *v);
-// A plan from the per-device cache, pinned: it is not evicted (freed) until
-// every pin is released with release_plan.  The batched executor holds its
-// plans pinned for its lifetime, a one-product call for the call.
-int cached_plan(int64_t K, int64_t bitsA, int64_t bitsB, const mgx_crt_plan **out);
-void release_plan(const mgx_crt_plan *pl);
-void plan_cache_stats(int64_t *total, int64_t *pinned);
-// C = A*B through a plan: stage launches on s, then one synchronization for
-// the device error word.
-int crt_matmul_with(const mgx_crt_plan *pl, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs,
-                    const mgx_limb *A, int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap, hipStream_t s);
-
 }  // namespace mgx

@@ -98,6 +98,38 @@ __global__ void mlp_adam_kernel(int n, float *p, const float *g, float *m, float
     p[i] -= lr_t * mi / (sqrtf(vi) + eps);
 }
 
+// One sample through all three layers in one workgroup (the dispatch's
+// inference: one feature vector per product).  The normalised input comes by
+// value; thread j computes hidden unit j of each layer (H <= 256).
+constexpr int PREDICT1_MAX_D = 32, PREDICT1_MAX_H = 256, PREDICT1_MAX_O = 64;
+struct Predict1In {
+    float x[PREDICT1_MAX_D];
+};
+__global__ __launch_bounds__(256) void mlp_predict1_kernel(const float *__restrict__ params, int D, int H, int O,
+                                                           Predict1In in, float *__restrict__ out)
+{
+    __shared__ float h1[PREDICT1_MAX_H], h2[PREDICT1_MAX_H];
+    const float *W1 = params, *b1 = W1 + H * D, *W2 = b1 + H, *b2 = W2 + H * H, *W3 = b2 + H, *b3 = W3 + O * H;
+    const int j = threadIdx.x;
+    if (j < H) {
+        float a = b1[j];
+        for (int i = 0; i < D; i++) a = fmaf(W1[j * D + i], in.x[i], a);
+        h1[j] = a > 0.f ? a : 0.f;
+    }
+    __syncthreads();
+    if (j < H) {
+        float a = b2[j];
+        for (int i = 0; i < H; i++) a = fmaf(W2[j * H + i], h1[i], a);
+        h2[j] = a > 0.f ? a : 0.f;
+    }
+    __syncthreads();
+    if (j < O) {
+        float a = b3[j];
+        for (int i = 0; i < H; i++) a = fmaf(W3[j * H + i], h2[i], a);
+        out[j] = a;
+    }
+}
+
 }  // namespace mgx
 
 using namespace mgx;
@@ -106,6 +138,7 @@ struct mgx_costmodel {
     int D, H, O;
     int64_t np;
     float *params = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;
+    float *d_out = nullptr, *h_out = nullptr;   // one-sample prediction: device result, pinned host copy
     int64_t step = 0;
     // feature normalisation (host): x' = (x - mean) / scale
     std::vector<float> mean, scale;
@@ -192,7 +225,11 @@ void normalise(const mgx_costmodel *cm, const float *X, int Bn, std::vector<floa
 
 }  // namespace
 
+namespace mgx { int costmodel_predict1(mgx_costmodel *cm, const float *x, float *y, hipStream_t s); }
+
 extern "C" {
+
+int mgx_costmodel_destroy(mgx_costmodel *cm);
 
 int mgx_costmodel_create(mgx_costmodel **out, int D, int H, int O, uint64_t seed)
 {
@@ -222,8 +259,10 @@ int mgx_costmodel_create(mgx_costmodel **out, int D, int H, int O, uint64_t seed
     for (int64_t i = 0; i < (int64_t)O * H; i++) p[o++] = a2 * rnd();
     size_t bytes = (size_t)cm->np * 4;
     if (hipMalloc(&cm->params, bytes) != hipSuccess || hipMalloc(&cm->grad, bytes) != hipSuccess ||
-        hipMalloc(&cm->m, bytes) != hipSuccess || hipMalloc(&cm->v, bytes) != hipSuccess) {
-        delete cm;
+        hipMalloc(&cm->m, bytes) != hipSuccess || hipMalloc(&cm->v, bytes) != hipSuccess ||
+        hipMalloc(&cm->d_out, 4 * PREDICT1_MAX_O) != hipSuccess ||
+        hipHostMalloc(&cm->h_out, 4 * PREDICT1_MAX_O, hipHostMallocDefault) != hipSuccess) {
+        mgx_costmodel_destroy(cm);
         return fail(MGX_ERR_OOM, "cost-model allocation");
     }
     MGX_HIP(hipMemcpy(cm->params, p.data(), bytes, hipMemcpyHostToDevice));
@@ -237,7 +276,8 @@ int mgx_costmodel_create(mgx_costmodel **out, int D, int H, int O, uint64_t seed
 int mgx_costmodel_destroy(mgx_costmodel *cm)
 {
     if (!cm) return MGX_OK;
-    hipFree(cm->params); hipFree(cm->grad); hipFree(cm->m); hipFree(cm->v);
+    hipFree(cm->params); hipFree(cm->grad); hipFree(cm->m); hipFree(cm->v); hipFree(cm->d_out);
+    if (cm->h_out) hipHostFree(cm->h_out);
     delete cm;
     return MGX_OK;
 }
@@ -359,6 +399,8 @@ int mgx_costmodel_train(mgx_costmodel *cm, const float *X, const float *T, const
 
 int mgx_costmodel_predict(mgx_costmodel *cm, const float *X, int Bn, float *Y)
 {
+    if (Bn == 1 && cm->D <= PREDICT1_MAX_D && cm->H <= PREDICT1_MAX_H && cm->O <= PREDICT1_MAX_O)
+        return costmodel_predict1(cm, X, Y, nullptr);   // one sample: the fused kernel
     hipStream_t s = nullptr;
     Work w;
     MGX_TRY(alloc_work(cm, Bn, w, s));
@@ -373,3 +415,21 @@ int mgx_costmodel_predict(mgx_costmodel *cm, const float *X, int Bn, float *Y)
 }
 
 }  // extern "C"
+
+namespace mgx {
+// The dispatch's inference for one raw feature vector x (normalised here),
+// on the caller's stream: one launch, one 4 O-byte copy, one synchronization.
+int costmodel_predict1(mgx_costmodel *cm, const float *x, float *y, hipStream_t s)
+{
+    if (cm->D > PREDICT1_MAX_D || cm->H > PREDICT1_MAX_H || cm->O > PREDICT1_MAX_O)
+        return mgx_costmodel_predict(cm, x, 1, y);
+    Predict1In in{};
+    for (int d = 0; d < cm->D; d++) in.x[d] = (x[d] - cm->mean[d]) / cm->scale[d];
+    hipLaunchKernelGGL(mlp_predict1_kernel, dim3(1), dim3(256), 0, s, cm->params, cm->D, cm->H, cm->O, in, cm->d_out);
+    MGX_HIP(hipGetLastError());
+    MGX_HIP(hipMemcpyAsync(cm->h_out, cm->d_out, (size_t)cm->O * 4, hipMemcpyDeviceToHost, s));
+    MGX_HIP(hipStreamSynchronize(s));
+    memcpy(y, cm->h_out, (size_t)cm->O * 4);
+    return MGX_OK;
+}
+}  // namespace mgx

@@ -10,6 +10,7 @@
 #include <vector>
 
 namespace mgx {
+int costmodel_predict1(mgx_costmodel *cm, const float *x, float *y, hipStream_t s);
 int plain_mul(bool strided, bool addmul, int64_t K, int64_t M, int64_t N, uint64_t *C, int64_t Crs, const uint64_t *A,
               int64_t Ars, const uint64_t *B, int64_t Brs, int64_t cap, int64_t opwords, hipStream_t s);
 int squarify_with(int method, int64_t K, int64_t M, int64_t N, uint64_t *C, int64_t Crs, const uint64_t *A,
@@ -55,28 +56,75 @@ __global__ void random_device_kernel(uint64_t *M, int64_t Wd, int64_t H, int64_t
     }
 }
 
-// per-matrix statistics: hist[6], sum of bits, max bits, sum of 2048-bit blocks
-__global__ void feature_scan_kernel(const uint64_t *X, int64_t rs, int64_t Wn, int64_t rows, int64_t cols,
-                                    unsigned long long *acc)
+// Statistics of up to two matrices in one launch (workgroups [0, nblk0)
+// scan the first), with the loads of four entries in flight per thread: the
+// dispatch's one pass over the inputs, whose maximum bit lengths then also
+// size the multi-modular plan.  Bit lengths are limb-granular (64 |size|,
+// from the size words alone: one cache line per entry).
+constexpr int STAT_PITCH = 16;   // 64-bit counters 128 bytes apart
+struct StatArgs {
+    const uint64_t *X[2];
+    int64_t rs[2], rows[2], cols[2];
+    int nblk0;
+};
+__global__ __launch_bounds__(256) void feature_scan2_kernel(StatArgs sa, int64_t Wn, unsigned long long *acc_pair)
 {
-    const int64_t n = rows * cols;
+    const int m = (int)blockIdx.x < sa.nblk0 ? 0 : 1;
+    const int64_t blk = m ? blockIdx.x - sa.nblk0 : blockIdx.x, nblk = m ? gridDim.x - sa.nblk0 : sa.nblk0;
+    const uint64_t *X = sa.X[m];
+    const int64_t rs = sa.rs[m], cols = sa.cols[m], n = sa.rows[m] * cols, stride = nblk * blockDim.x;
+    // counter c of matrix m sits alone in a 128-byte line: atomics to one
+    // line serialize in one L2 channel
+    unsigned long long *acc = acc_pair + STAT_PITCH * 9 * m;
+    __shared__ unsigned long long s_part[4][9];
     unsigned long long h[6] = {0, 0, 0, 0, 0, 0}, sb = 0, mb = 0, sk = 0;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t *p = X + ((e / cols) * rs + e % cols) * Wn;
-        int64_t s = (int64_t)p[Wn - 1], a = s < 0 ? -s : s;
-        while (a > 0 && p[a - 1] == 0) a--;
-        int64_t b = a ? (a - 1) * 64 + 64 - __clzll(p[a - 1]) : 0;
-        int bin = b <= 128 ? 0 : b <= 512 ? 1 : b <= 2048 ? 2 : b <= 8192 ? 3 : b <= 32768 ? 4 : 5;
+    for (int64_t e0 = blk * (int64_t)blockDim.x + threadIdx.x; e0 < n; e0 += 4 * stride) {
+        const uint64_t *p[4];
+        int64_t a[4];
 #pragma unroll
-        for (int i = 0; i < 6; i++) h[i] += bin == i;
-        sb += b;
-        mb = b > (int64_t)mb ? b : mb;
-        sk += (b + 2047) / 2048;
+        for (int j = 0; j < 4; j++) {
+            const int64_t e = e0 + j * stride;
+            p[j] = X + ((e / cols) * rs + e % cols) * Wn;
+            const int64_t sz = e < n ? (int64_t)p[j][Wn - 1] : 0;
+            a[j] = min(sz < 0 ? -sz : sz, Wn - 1);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (e0 + j * stride >= n) continue;
+            const int64_t b = 64 * a[j];   // limb-granular bit length, from the size word alone
+            const int bin = b <= 128 ? 0 : b <= 512 ? 1 : b <= 2048 ? 2 : b <= 8192 ? 3 : b <= 32768 ? 4 : 5;
+#pragma unroll
+            for (int i = 0; i < 6; i++) h[i] += bin == i;
+            sb += b;
+            mb = b > (int64_t)mb ? b : mb;
+            sk += (b + 2047) / 2048;
+        }
     }
-    for (int i = 0; i < 6; i++) atomicAdd(&acc[i], h[i]);
-    atomicAdd(&acc[6], sb);
-    atomicMax(&acc[7], mb);
-    atomicAdd(&acc[8], sk);
+    // wave sums first: one atomic per wave and counter
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+        for (int i = 0; i < 6; i++) h[i] += __shfl_xor(h[i], o);
+        sb += __shfl_xor(sb, o);
+        sk += __shfl_xor(sk, o);
+        mb = max(mb, (unsigned long long)__shfl_xor(mb, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        const int w = threadIdx.x >> 6;
+        for (int i = 0; i < 6; i++) s_part[w][i] = h[i];
+        s_part[w][6] = sb;
+        s_part[w][7] = mb;
+        s_part[w][8] = sk;
+    }
+    __syncthreads();
+    if (threadIdx.x < 9) {   // one atomic per counter per workgroup
+        const int c = threadIdx.x;
+        unsigned long long v = s_part[0][c];
+        for (int w = 1; w < 4; w++) v = c == 7 ? max(v, s_part[w][c]) : v + s_part[w][c];
+        if (v) {
+            if (c == 7) atomicMax(&acc[STAT_PITCH * c], v);
+            else atomicAdd(&acc[STAT_PITCH * c], v);
+        }
+    }
 }
 
 }  // namespace mgx
@@ -93,26 +141,58 @@ int plain_route(int route, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t
                          route == MGX_ROUTE_WINOGRAD1 ? 1 : 2, s);
 }
 
+// per matrix: bit-length histogram (6 bins), sum of bits, max bits, sum of 2048-bit blocks
 struct Stats {
     unsigned long long v[9];
 };
 
-int scan_stats(int64_t rows, int64_t cols, const mgx_limb *X, int64_t Xrs, int64_t cap, Stats &st, hipStream_t s)
+// Statistics of A (K x M) and B (M x N) with one launch and one
+// synchronization.
+int scan_stats_pair(int64_t K, int64_t M, int64_t N, const mgx_limb *A, int64_t Ars, const mgx_limb *B, int64_t Brs,
+                    int64_t cap, Stats &sa_out, Stats &sb_out, hipStream_t s)
 {
     unsigned long long *d = nullptr;
-    MGX_TRY(scratch_alloc((void **)&d, sizeof(Stats), s));
-    MGX_HIP(hipMemsetAsync(d, 0, sizeof(Stats), s));
-    int64_t n = rows * cols;
-    if (n > 0) {
-        int grid = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(n, 256), 1), 2048);
-        hipLaunchKernelGGL(feature_scan_kernel, dim3(grid), dim3(256), 0, s, (const uint64_t *)X, Xrs, cap + 1, rows,
-                           cols, d);
+    const size_t bytes = 2 * 9 * STAT_PITCH * 8;
+    MGX_TRY(scratch_alloc((void **)&d, bytes, s));
+    MGX_HIP(hipMemsetAsync(d, 0, bytes, s));
+    StatArgs sa{};
+    const int64_t rows[2] = {K, M}, cols[2] = {M, N}, rs[2] = {Ars, Brs};
+    const mgx_limb *X[2] = {A, B};
+    int grid[2] = {0, 0};
+    for (int m = 0; m < 2; m++) {
+        const int64_t n = rows[m] * cols[m];
+        sa.X[m] = (const uint64_t *)X[m];
+        sa.rs[m] = rs[m];
+        sa.rows[m] = rows[m];
+        sa.cols[m] = cols[m];
+        grid[m] = n > 0 ? (int)std::min<int64_t>(ceil_div(n, 1024), 2048) : 0;
     }
-    MGX_HIP(hipMemcpyAsync(&st, d, sizeof(Stats), hipMemcpyDeviceToHost, s));
+    sa.nblk0 = grid[0];
+    if (grid[0] + grid[1] > 0)
+        hipLaunchKernelGGL(feature_scan2_kernel, dim3(grid[0] + grid[1]), dim3(256), 0, s, sa, cap + 1, d);
+    MGX_HIP(hipGetLastError());
+    unsigned long long h[2 * 9 * STAT_PITCH];
+    MGX_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, s));
     MGX_TRY(scratch_free(d, s));
     MGX_HIP(hipStreamSynchronize(s));
+    for (int c = 0; c < 9; c++) {
+        sa_out.v[c] = h[STAT_PITCH * c];
+        sb_out.v[c] = h[STAT_PITCH * (9 + c)];
+    }
     return MGX_OK;
 }
+
+int features_of(int64_t K, int64_t M, int64_t N, const Stats &a, const Stats &b, float *f)
+{
+    int64_t hist[6];
+    for (int i = 0; i < 6; i++) hist[i] = (int64_t)(a.v[i] + b.v[i]);
+    double cnt = (double)(K * M + M * N);
+    if (cnt <= 0) cnt = 1;
+    return mgx_tune_features(K, M, N, hist, (double)(a.v[6] + b.v[6]) / cnt, (int64_t)std::max(a.v[7], b.v[7]),
+                             (double)(a.v[8] + b.v[8]) / cnt, f);
+}
+
+int choose_route(int64_t K, int64_t M, int64_t N, const float *f, int *route, hipStream_t s);
 
 mgx_costmodel *g_model = nullptr;
 std::mutex g_mu;
@@ -161,33 +241,40 @@ int mgx_tune_features_device(int64_t K, int64_t M, int64_t N, const mgx_limb *A,
 {
     MGX_TRY(ensure_init());
     Stats a, b;
-    MGX_TRY(scan_stats(K, M, A, Ars, cap, a, nullptr));
-    MGX_TRY(scan_stats(M, N, B, Brs, cap, b, nullptr));
-    int64_t hist[6];
-    for (int i = 0; i < 6; i++) hist[i] = (int64_t)(a.v[i] + b.v[i]);
-    double cnt = (double)(K * M + M * N);
-    if (cnt <= 0) cnt = 1;
-    return mgx_tune_features(K, M, N, hist, (double)(a.v[6] + b.v[6]) / cnt, (int64_t)std::max(a.v[7], b.v[7]),
-                             (double)(a.v[8] + b.v[8]) / cnt, f);
+    MGX_TRY(scan_stats_pair(K, M, N, A, Ars, B, Brs, cap, a, b, nullptr));
+    return features_of(K, M, N, a, b, f);
 }
 
 int mgx_matmul_route(int route, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A,
                      int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap, void *stream)
 {
     MGX_TRY(ensure_init());
+    if (K < 0 || M < 0 || N < 0 || cap < 1) return fail(MGX_ERR_SHAPE, "bad shape");
+    if (route != MGX_ROUTE_AUTO && (route < 0 || route >= MGX_ROUTES)) return fail(MGX_ERR_ARG, "unknown route %d", route);
     hipStream_t s = as_stream(stream);
-    if (route == MGX_ROUTE_AUTO) MGX_TRY(mgx_dispatch_choose(K, M, N, A, Ars, B, Brs, cap, &route));
-    if (route == MGX_ROUTE_CRT) return mgx_crt_mfma_matmul(nullptr, K, M, N, C, Crs, A, Ars, B, Brs, cap, stream);
-    if (route < 0 || route >= MGX_ROUTES) return fail(MGX_ERR_ARG, "unknown route %d", route);
     if (K == 0 || N == 0) return MGX_OK;
-    int64_t ba = 1, bb = 1;
-    MGX_TRY(mgx_crt_scan_bits(K, M, A, Ars, cap, &ba, stream));
-    MGX_TRY(mgx_crt_scan_bits(M, N, B, Brs, cap, &bb, stream));
     if (M == 0) {
         MGX_HIP(hipMemset2DAsync(C, (size_t)(Crs * (cap + 1) * 8), 0, (size_t)(N * (cap + 1) * 8), (size_t)K, s));
-    } else {
-        MGX_TRY(plain_route(route, K, M, N, C, Crs, A, Ars, B, Brs, cap, std::max<int64_t>(std::max(ba, bb), 1), s));
+        return check_device_error(s);
     }
+    // one pass over the inputs: the features for the dispatch and the bit
+    // bounds for whichever route runs
+    Stats sa, sb;
+    MGX_TRY(scan_stats_pair(K, M, N, A, Ars, B, Brs, cap, sa, sb, s));
+    const int64_t ba = std::max<int64_t>((int64_t)sa.v[7], 1), bb = std::max<int64_t>((int64_t)sb.v[7], 1);
+    if (route == MGX_ROUTE_AUTO) {
+        float f[MGX_FEATURES];
+        MGX_TRY(features_of(K, M, N, sa, sb, f));
+        MGX_TRY(choose_route(K, M, N, f, &route, s));
+    }
+    if (route == MGX_ROUTE_CRT) {
+        const mgx_crt_plan *pl = nullptr;
+        MGX_TRY(cached_plan(M, ba, bb, &pl));
+        int st = crt_matmul_with(pl, K, M, N, C, Crs, A, Ars, B, Brs, cap, s);
+        release_plan(pl);
+        return st;
+    }
+    MGX_TRY(plain_route(route, K, M, N, C, Crs, A, Ars, B, Brs, cap, std::max(ba, bb), s));
     return check_device_error(s);
 }
 
@@ -249,6 +336,17 @@ int mgx_dispatch_choose(int64_t K, int64_t M, int64_t N, const mgx_limb *A, int6
 {
     float f[MGX_FEATURES];
     MGX_TRY(mgx_tune_features_device(K, M, N, A, Ars, B, Brs, cap, f));
+    return choose_route(K, M, N, f, route, nullptr);
+}
+
+}  // extern "C"
+
+namespace {
+// The route for a product's features: the cost model's prediction when a
+// model is installed, else the exhaustive autotuner's remembered choice for
+// the feature bucket.
+int choose_route(int64_t K, int64_t M, int64_t N, const float *f, int *route, hipStream_t s_pred)
+{
     mgx_costmodel *cm;
     {
         std::lock_guard<std::mutex> lk(g_mu);
@@ -256,7 +354,7 @@ int mgx_dispatch_choose(int64_t K, int64_t M, int64_t N, const mgx_limb *A, int6
     }
     if (cm) {
         float y[MGX_ROUTES];
-        MGX_TRY(mgx_costmodel_predict(cm, f, 1, y));
+        MGX_TRY(costmodel_predict1(cm, f, y, s_pred));
         // the plain kernel takes operands of at most 512 32-bit words, and
         // each Winograd level adds up to two bits to them: routes that
         // cannot take the product are never chosen, whatever the prediction
@@ -302,4 +400,4 @@ int mgx_dispatch_choose(int64_t K, int64_t M, int64_t N, const mgx_limb *A, int6
     return MGX_OK;
 }
 
-}  // extern "C"
+}  // namespace

@@ -118,6 +118,7 @@ def _declare(L):
         "mgx_costmodel_load": [ctypes.POINTER(c_vp), ctypes.c_char_p],
         "mgx_dispatch_set_model": [c_vp],
         "mgx_dispatch_choose": [c_i64, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_i64, ctypes.POINTER(ctypes.c_int)],
+        "mgx_measure_peaks": [ctypes.POINTER(ctypes.c_double)] * 3,
         "mgx_batch_create": [ctypes.POINTER(c_vp), c_vp, c_i64, c_vp],
         "mgx_batch_run": [c_vp, c_vp],
         "mgx_batch_route": [c_vp, c_i64],

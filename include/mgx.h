@@ -193,8 +193,9 @@ int mgx_crt_plan_destroy(mgx_crt_plan *plan);
 int64_t mgx_crt_plan_primes(const mgx_crt_plan *plan);
 int64_t mgx_crt_plan_slices(const mgx_crt_plan *plan);   /* S: 2 (primes < 65280) or 4 (primes < 2^31) */
 int mgx_crt_plan_prime_list(const mgx_crt_plan *plan, uint32_t *out);
-/* C = A*B through the plan (plan NULL: scan the inputs' bit lengths on the
- * device, which synchronizes the stream once, and plan internally).  With a
+/* C = A*B through the plan (plan NULL: scan the inputs' limb counts on the
+ * device, which synchronizes the stream once, and plan internally for
+ * |A| < 2^(64 max|size_A|), |B| < 2^(64 max|size_B|)).  With a
  * caller's plan, M must not exceed its K and every |A| < 2^bitsA, |B| < 2^bitsB
  * (checked on the device: MGX_ERR_SHAPE otherwise, C undefined). */
 int mgx_crt_mfma_matmul(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs,

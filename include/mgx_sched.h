@@ -43,7 +43,8 @@ int mgx_random_matrix_device(mgx_limb *M, int64_t W, int64_t H, int64_t rowstrid
  * <=128, <=512, <=2048, <=8192, <=32768, larger). */
 int mgx_tune_features(int64_t K, int64_t M, int64_t N, const int64_t *hist6, double mean_bits, int64_t max_bits,
                       double mean_blocks, float *out);
-/* Features of a device-resident product (scans the entries; synchronizes). */
+/* Features of a device-resident product (scans the entries' size words, so
+ * bit lengths are limb-granular, 64 |size|; synchronizes). */
 int mgx_tune_features_device(int64_t K, int64_t M, int64_t N, const mgx_limb *A, int64_t Ars, const mgx_limb *B,
                              int64_t Brs, int64_t cap, float *out);
 /* Times every route on the given device matrices (C is scratch), median of
@@ -75,6 +76,14 @@ int mgx_dispatch_set_model(mgx_costmodel *cm);
 /* The route MGX_ROUTE_AUTO would take for a device-resident product. */
 int mgx_dispatch_choose(int64_t K, int64_t M, int64_t N, const mgx_limb *A, int64_t Ars, const mgx_limb *B,
                         int64_t Brs, int64_t cap, int *route);
+
+/* ---------------- measured ceilings (no reference counterpart) ---------------- */
+/* Measures, on the current device, the INT8 MFMA issue rate
+ * (v_mfma_i32_32x32x32_i8, operations/s), the 32x32+64-bit integer
+ * multiply-add rate (v_mad_u64_u32, the plain route's limb product, per
+ * second) and the device copy bandwidth (bytes read + written per second).
+ * Any output pointer may be NULL.  Synchronous; takes 2 GiB of scratch. */
+int mgx_measure_peaks(double *int8_ops_per_s, double *mad_per_s, double *copy_bytes_per_s);
 
 /* ---------------- batched executor ---------------- */
 typedef struct {

@@ -216,6 +216,12 @@ def test_committed_model_loads_and_dispatches(gpu):
     mgx.check(L.mgx_costmodel_predict(cm, X.ctypes.data, 17, Y1.ctypes.data))
     mgx.check(L.mgx_costmodel_predict(cm2, X.ctypes.data, 17, Y2.ctypes.data))
     assert np.allclose(Y1, Y2, rtol=1e-6, atol=1e-5)
+    # one sample at a time takes the fused three-layer kernel: FP32 sums in a
+    # different order, so agreement to FP32 rounding of the batched kernels
+    for b in range(17):
+        y1 = np.zeros(d["O"], dtype=np.float32)
+        mgx.check(L.mgx_costmodel_predict(cm, np.ascontiguousarray(X[b]).ctypes.data, 1, y1.ctypes.data))
+        assert np.allclose(y1, Y1[b], rtol=1e-5, atol=1e-4), b
     try:
         mgx.check(L.mgx_dispatch_set_model(cm))
         K, M, N, bits, cap = 19, 23, 17, 900, 32
