@@ -207,7 +207,8 @@ const char *mgx_version(void) { return "mgx 0.1 gfx950"; }
 
 int mgx_set_param(int which, int64_t value)
 {
-    if (which < 0 || which > MGX_PARAM_CRT_SLICES) return fail(MGX_ERR_ARG, "unknown parameter %d", which);
+    if (which < 0 || which > MGX_PARAM_TRANSFER) return fail(MGX_ERR_ARG, "unknown parameter %d", which);
+    if (which == MGX_PARAM_TRANSFER && value < 0) return fail(MGX_ERR_ARG, "transfer size must be >= 0");
     if (which == MGX_PARAM_CRT_SLICES && value != 0 && value != 2 && value != 4)
         return fail(MGX_ERR_ARG, "CRT slice count must be 0, 2 or 4");
     if (which == MGX_PARAM_LEAF && value < 1) return fail(MGX_ERR_ARG, "leaf size must be >= 1");

@@ -10,6 +10,7 @@
 // nodes), is captured once and replayed with hipGraphLaunch.
 #include "crt_kernels.h"
 #include "plain_kernels.h"
+#include "queue.h"
 #include "../../include/mgx_sched.h"
 #include <algorithm>
 #include <vector>
@@ -62,16 +63,7 @@ __device__ __forceinline__ bool next_item(const Queue &q, int &job, int64_t &til
     __syncthreads();
     const int64_t it = s_item;
     __syncthreads();
-    if (it >= q.prefix[q.njobs]) return false;
-    int lo = 0, hi = q.njobs - 1;   // last job with prefix <= it
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (q.prefix[mid] <= it) lo = mid;
-        else hi = mid - 1;
-    }
-    job = lo;
-    tile = it - q.prefix[lo];
-    return true;
+    return queue_map(q.prefix, q.njobs, it, &job, &tile);
 }
 
 __global__ void reset_counters_kernel(unsigned long long *c, int n)
@@ -181,8 +173,16 @@ __global__ __launch_bounds__(256) void batch_plain_kernel(Queue q, const PlainJo
 
 using namespace mgx;
 
+struct BatchQueue {
+    int kind;        // mgx_batch_queue_info kinds
+    int slot;        // counter index
+    int64_t items;   // tiles over all jobs of the queue
+    int grid;        // workgroups of its persistent launch
+};
 struct mgx_batch {
     std::vector<int> routes;
+    std::vector<BatchQueue> queues;   // in launch order
+    unsigned long long *counters = nullptr;
     std::vector<const mgx_crt_plan *> plans;   // pinned in the plan cache while the batch lives
     void *scratch = nullptr;       // residues, Y, column sums, job tables, prefixes, counters
     int64_t scratch_bytes = 0;
@@ -402,6 +402,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
     // Counter slots: 0-1 entry-resident reduce (256 bytes), 2-5 residue GEMM, 6-7 reconstruct GEMM, 8 carry,
     // 9-14 plain classes, 15-16 generic reduce, 17-18 entry-resident reduce (512 bytes).
     unsigned long long *ctr = (unsigned long long *)(base + o_ctr);
+    b->counters = ctr;
     unsigned *err = device_error_word();
     const int toom = (int)param(MGX_PARAM_TOOM22);
     auto queue = [&](int slot, int64_t o_prefix, size_t njobs) {
@@ -420,6 +421,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             if (pj[c].empty()) continue;
             Queue q = queue(9 + c, o_pp[c], pj[c].size());
             int grid = (int)std::min<int64_t>(pp[c].back(), 2048);
+            b->queues.push_back({MGX_QUEUE_PLAIN, 9 + c, pp[c].back(), grid});
             const PlainJob *jobs = (const PlainJob *)(base + o_pj[c]);
             switch (NB_CLASSES[c]) {
             case 1: launch_plain<1>(grid, cs, q, jobs, toom, err); break;
@@ -438,12 +440,14 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
                       : kind == 2 ? (two ? batch_reduce_res_kernel<2, 8> : batch_reduce_res_kernel<4, 8>)
                                   : (two ? batch_reduce_res_kernel<2, 4> : batch_reduce_res_kernel<4, 4>);
             const int slot = kind == 0 ? two : kind == 1 ? 15 + two : 17 + two;
+            b->queues.push_back({MGX_QUEUE_REDUCE, slot, rp[c].back(), grid});
             hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, cs, queue(slot, o_rp[c], rj[c].size()),
                                (const ReduceJob *)(base + o_rj[c]), err);
         }
         for (int c = 0; c < 4; c++) {
             if (gj[c].empty()) continue;
             int grid = (int)std::min<int64_t>(gp[c].back(), 256);
+            b->queues.push_back({MGX_QUEUE_RESIDUE_GEMM, 2 + c, gp[c].back(), grid});
             auto kern = c == 0 ? batch_gemm_kernel<false> : c == 1 ? batch_gemm_kernel<true>
                       : c == 2 ? batch_gemm2_kernel<false> : batch_gemm2_kernel<true>;
             hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, cs, queue(2 + c, o_gp[c], gj[c].size()),
@@ -452,11 +456,13 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
         for (int c = 0; c < 2; c++) {
             if (cj[c].empty()) continue;
             int grid = (int)std::min<int64_t>(cp[c].back(), 1024);
+            b->queues.push_back({MGX_QUEUE_RECONSTRUCT, 6 + c, cp[c].back(), grid});
             hipLaunchKernelGGL(c ? batch_recon_gemm_kernel<2> : batch_recon_gemm_kernel<4>, dim3(grid), dim3(256), 0,
                                cs, queue(6 + c, o_cp[c], cj[c].size()), (const ReconJob *)(base + o_cj[c]));
         }
         if (!kj.empty()) {
             int grid = (int)std::min<int64_t>(kp.back(), 1024);
+            b->queues.push_back({MGX_QUEUE_CARRY, 8, kp.back(), grid});
             hipLaunchKernelGGL(batch_recon_carry_kernel, dim3(grid), dim3(256), 0, cs, queue(8, o_kp, kj.size()),
                                (const ReconJob *)(base + o_kj), err);
         }
@@ -487,6 +493,30 @@ int mgx_batch_route(const mgx_batch *b, int64_t i)
 }
 
 int64_t mgx_batch_scratch_bytes(const mgx_batch *b) { return b ? b->scratch_bytes : -1; }
+
+int64_t mgx_batch_queue_count(const mgx_batch *b) { return b ? (int64_t)b->queues.size() : -1; }
+
+int mgx_batch_queue_info(const mgx_batch *b, int64_t i, int *kind, int64_t *items, int *grid, int64_t *claims)
+{
+    if (!b || i < 0 || i >= (int64_t)b->queues.size()) return fail(MGX_ERR_ARG, "no queue %lld", (long long)i);
+    const BatchQueue &q = b->queues[i];
+    if (kind) *kind = q.kind;
+    if (items) *items = q.items;
+    if (grid) *grid = q.grid;
+    if (claims) {
+        unsigned long long c = 0;
+        MGX_HIP(hipDeviceSynchronize());
+        MGX_HIP(hipMemcpy(&c, b->counters + q.slot, sizeof c, hipMemcpyDeviceToHost));
+        *claims = (int64_t)c;
+    }
+    return MGX_OK;
+}
+
+int mgx_queue_map(const int64_t *prefix, int njobs, int64_t item, int *job, int64_t *tile)
+{
+    if (!prefix || !job || !tile) return fail(MGX_ERR_ARG, "null argument");
+    return queue_map(prefix, njobs, item, job, tile) ? 0 : 1;
+}
 
 int mgx_batch_destroy(mgx_batch *b)
 {

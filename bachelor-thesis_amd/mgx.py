@@ -123,6 +123,9 @@ def _declare(L):
         "mgx_batch_run": [c_vp, c_vp],
         "mgx_batch_route": [c_vp, c_i64],
         "mgx_batch_destroy": [c_vp],
+        "mgx_batch_queue_info": [c_vp, c_i64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(c_i64),
+                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(c_i64)],
+        "mgx_queue_map": [c_vp, ctypes.c_int, c_i64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(c_i64)],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
@@ -136,6 +139,8 @@ def _declare(L):
     L.mgx_costmodel_param_count.restype = c_i64
     L.mgx_batch_scratch_bytes.argtypes = [c_vp]
     L.mgx_batch_scratch_bytes.restype = c_i64
+    L.mgx_batch_queue_count.argtypes = [c_vp]
+    L.mgx_batch_queue_count.restype = c_i64
     L.mgx_entry_to_hex.restype = c_i64
     L.mgx_get_param.restype = c_i64
     L.mgx_crt_plan_primes.argtypes = [c_vp]

@@ -173,7 +173,13 @@ int mgx_squarify_mul(int method, int64_t K, int64_t M, int64_t N, mgx_limb *C, i
 /* Host-memory entry: copies A and B to the device, multiplies with the
  * mat-gpu method code (1, 2, 3, 35, 37, 357, 4, 47 as in
  * mat-gpu/main.cu:358-502; 8 = CRT-MFMA; 0 = cost-model choice), copies C
- * back.  Replaces the *_pregpu layer (mat-gpu/mat-pregpu.h:6-63). */
+ * back.  Replaces the *_pregpu layer (mat-gpu/mat-pregpu.h:6-63).
+ * Out of core (MGX_PARAM_TRANSFER = T > 0 and a side above T): square
+ * products of methods 3/35/4/37/357/47 recurse over 2 x 2 blocks in host
+ * memory with the method's own sequence until a block side is at most T
+ * (as transfer_recurse, mat-gpu/mat-pregpu.cu:10-96), block additions run
+ * on the device over staged copies; other shapes and methods are tiled in
+ * T x T blocks whose products accumulate on the device. */
 int mgx_matmul_host(int method, int64_t K, int64_t M, int64_t N, mgx_limb *C, const mgx_limb *A,
                     const mgx_limb *B, int64_t cap, double *seconds);
 
@@ -269,8 +275,11 @@ typedef enum {
     MGX_PARAM_LEAF = 0,          /* recursion leaf size (reference BASE_SIZE = 256) */
     MGX_PARAM_TOOM22 = 1,        /* Toom-22 block cutoff in 64-limb blocks (0 = never) */
     MGX_PARAM_WINOGRAD_DEPTH = 2, /* max recursion levels for the plain route (-1 = to the leaf) */
-    MGX_PARAM_CRT_SLICES = 3      /* CRT digits per residue for new plans: 0 = 2 when primes below 65280
+    MGX_PARAM_CRT_SLICES = 3,     /* CRT digits per residue for new plans: 0 = 2 when primes below 65280
                                      suffice, else 4 (default); 2 or 4 to force */
+    MGX_PARAM_TRANSFER = 4        /* mgx_matmul_host: largest block side moved to the device at once
+                                     (reference TRANSFER_SIZE = 512, mat-gpu/mat-params.h:6); 0 = the whole
+                                     product (default) */
 } mgx_param;
 int mgx_set_param(int param, int64_t value);
 int64_t mgx_get_param(int param);

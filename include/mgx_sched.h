@@ -110,6 +110,23 @@ int mgx_batch_route(const mgx_batch *b, int64_t i);
 int64_t mgx_batch_scratch_bytes(const mgx_batch *b);
 int mgx_batch_destroy(mgx_batch *b);
 
+/* Work queues of a batch, in launch order (one per persistent launch).  A
+ * queue is the concatenation of its jobs' tile ranges; workgroups claim
+ * items from one counter until a claim falls past the end, so after a
+ * replay every queue's counter reads items + grid (claims).  Reading the
+ * claims synchronizes the device.  Diagnostics with no reference counterpart
+ * (the reference's host scheduler, mat-gpu/job.cu:183-336, has none). */
+typedef enum {
+    MGX_QUEUE_PLAIN = 0, MGX_QUEUE_REDUCE = 1, MGX_QUEUE_RESIDUE_GEMM = 2, MGX_QUEUE_RECONSTRUCT = 3,
+    MGX_QUEUE_CARRY = 4
+} mgx_queue_kind;
+int64_t mgx_batch_queue_count(const mgx_batch *b);
+int mgx_batch_queue_info(const mgx_batch *b, int64_t i, int *kind, int64_t *items, int *grid, int64_t *claims);
+/* The queues' item mapping (host copy of the device code): item -> (job,
+ * tile) for prefix[0] = 0 <= ... <= prefix[njobs] = items.  Returns 0, or 1
+ * when item is past the end (the claim that ends a workgroup's loop). */
+int mgx_queue_map(const int64_t *prefix, int njobs, int64_t item, int *job, int64_t *tile);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,9 +64,23 @@ def run_and_check(mgx, L, bufs, prods, wants, reps=2):
                 got = dC.to_array((K, N, cap + 1))
                 assert to_ints(got) == want, (rep, i, prods[i].K, prods[i].M, prods[i].N)
                 assert is_normalized(got)
+            check_queues(mgx, L, b)
         return routes
     finally:
         L.mgx_batch_destroy(b)
+
+
+def check_queues(mgx, L, b):
+    """Every persistent launch drained its queue: each item claimed once and
+    each workgroup left after one failing claim (counter = items + grid)."""
+    nq = L.mgx_batch_queue_count(b)
+    assert nq > 0
+    for q in range(nq):
+        kind, items, grid, claims = ctypes.c_int(), i64(), ctypes.c_int(), i64()
+        mgx.check(L.mgx_batch_queue_info(b, q, ctypes.byref(kind), ctypes.byref(items), ctypes.byref(grid),
+                                         ctypes.byref(claims)))
+        assert 0 < grid.value <= max(items.value, 1)
+        assert claims.value == items.value + grid.value, (q, kind.value, items.value, grid.value, claims.value)
 
 
 def test_batch_every_class_two_digits(gpu):
