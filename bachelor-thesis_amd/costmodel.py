@@ -1,17 +1,20 @@
-"""Train and evaluate the learned route dispatcher.
+"""Train and evaluate the learned schedule dispatcher.
 
     python bachelor-thesis_amd/costmodel.py gen    --data timings.npz [--configs 5000]
     python bachelor-thesis_amd/costmodel.py train  [--data timings.npz] [--configs 5000] [--out costmodel_params.json]
     python bachelor-thesis_amd/costmodel.py eval   [--params costmodel_params.json] [--json out.json]
 
-`train` times every route (mgx_tune_time_routes: the exhaustive autotuner) on
-seeded synthetic products of random shape and bit-length mix, then fits the
-FP32 MLP (3 layers, 64 hidden) with the library's own HIP forward/backward/Adam
-kernels to predict log2 of each route's time from the product's features.
-`eval` runs held-out heavy-tailed mixes, compares the network's route with the
-exhaustive choice (regret = t(chosen)/t(best) - 1), checks that the chosen
-route's output is bit-identical to the other routes', and reports what the
-network gains over fixed dispatch and over tuning at run time.
+`gen` times every schedule (mgx_tune_time_schedules: the exhaustive
+autotuner over route, digit count, residue-GEMM tile, Toom cutoff and
+Winograd depth) on seeded synthetic products of random shape and bit-length
+mix; `train` fits the FP32 MLP (3 layers, 64 hidden) with the library's own
+HIP forward/backward/Adam kernels to predict log2 of each schedule's time from
+the product's features.  `eval` runs held-out heavy-tailed mixes through
+MGX_ROUTE_AUTO with the model installed (so the dispatch's own scan and
+inference are in the timed cost), compares with the exhaustive best
+(regret = t(auto)/t(best) - 1), checks the output bit for bit against the
+default schedule, and reports what the network gains over fixed dispatch and
+over tuning at run time.
 
 The model only picks a schedule; every route is exact.
 """
@@ -30,8 +33,8 @@ sys.path.insert(0, HERE)
 import mgx  # noqa: E402
 
 vp, i64 = ctypes.c_void_p, ctypes.c_int64
-D, H, O = 13, 64, 4
-ROUTES = ["crt", "plain", "winograd1", "winograd2"]
+D, H, O = 13, 64, 9
+SCHEDULES = ["crt2", "crt2_t64", "crt4", "plain", "plain_notoom", "wino1", "wino2", "wino1_notoom", "wino2_notoom"]
 DEFAULT_PARAMS = os.path.join(HERE, "costmodel_params.json")
 
 
@@ -79,15 +82,34 @@ class Problem:
     def times(self, reps=1):
         t = np.zeros(O)
         est = ctypes.c_uint()
-        mgx.check(lib().mgx_tune_time_routes(self.K, self.M, self.N, self.C.ptr, self.A.ptr, self.B.ptr, self.cap,
-                                             reps, t.ctypes.data, ctypes.byref(est)))
+        mgx.check(lib().mgx_tune_time_schedules(self.K, self.M, self.N, self.C.ptr, self.A.ptr, self.B.ptr, self.cap,
+                                                reps, t.ctypes.data, ctypes.byref(est)))
         return t, est.value
 
-    def run(self, route):
+    def run(self, schedule):
+        """Output of one schedule (MGX_ROUTE_AUTO = -1: the dispatch's choice)."""
         self.C.zero()
-        mgx.check(lib().mgx_matmul_route(route, self.K, self.M, self.N, self.C.ptr, self.N, self.A.ptr, self.M,
-                                         self.B.ptr, self.N, self.cap, None))
+        mgx.check(lib().mgx_matmul_schedule(schedule, self.K, self.M, self.N, self.C.ptr, self.N, self.A.ptr, self.M,
+                                            self.B.ptr, self.N, self.cap, None))
         return self.C.to_array((self.K, self.N, self.cap + 1))
+
+    def time_auto(self, reps=5):
+        """Median wall time of MGX_ROUTE_AUTO calls (scan, inference, product)."""
+        L = lib()
+        args = (self.K, self.M, self.N, self.C.ptr, self.N, self.A.ptr, self.M, self.B.ptr, self.N, self.cap, None)
+        mgx.check(L.mgx_matmul_schedule(-1, *args))
+        t = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            mgx.check(L.mgx_matmul_schedule(-1, *args))
+            t.append(time.perf_counter() - t0)
+        return float(np.median(t))
+
+    def chosen(self):
+        sc = ctypes.c_int()
+        mgx.check(lib().mgx_dispatch_choose_schedule(self.K, self.M, self.N, self.A.ptr, self.M, self.B.ptr, self.N,
+                                                     self.cap, ctypes.byref(sc)))
+        return sc.value
 
     def free(self):
         for b in (self.A, self.B, self.C):
@@ -166,7 +188,7 @@ def save_model(cm, mean, scale, path, meta):
     p = np.zeros(n, dtype=np.float32)
     mgx.check(L.mgx_costmodel_get_params(cm, p.ctypes.data))
     with open(path, "w") as f:
-        json.dump({"D": D, "H": H, "O": O, "routes": ROUTES, "mean": mean.tolist(), "scale": scale.tolist(),
+        json.dump({"D": D, "H": H, "O": O, "schedules": SCHEDULES, "mean": mean.tolist(), "scale": scale.tolist(),
                    "params": p.tolist(), "meta": meta}, f)
     # the same numbers as text for mgx_costmodel_load (C callers, mat-gpu -m 0)
     with open(os.path.splitext(path)[0] + ".txt", "w") as f:
@@ -215,34 +237,44 @@ HELD_OUT = [
 ]
 
 
-def evaluate(cm, reps=3, check_exact=True):
+def evaluate(cm, reps=3, check_exact=True, mixes=None):
+    """Held-out mixes through MGX_ROUTE_AUTO with `cm` installed."""
+    L = lib()
+    mgx.check(L.mgx_dispatch_set_model(cm))
     rows = []
-    for i, (name, K, M, N, blo, bhi, frac) in enumerate(HELD_OUT):
-        p = Problem(K, M, N, blo, bhi, frac, 777 + i)
-        f = p.features()
-        y = predict(cm, f[None, :])[0]
-        choice = int(np.argmin(y))
-        t, est = p.times(reps=reps)
-        best = int(np.argmin(t))
-        regret = t[choice] / t[best] - 1.0
-        exact = None
-        if check_exact:
-            out = p.run(choice)
-            other = p.run(0 if choice != 0 else (1 if t[1] < 1e29 and not (est & 2) else 0))
-            exact = bool((out == other).all()) if True else None
-        rows.append({"mix": name, "K": K, "M": M, "N": N, "choice": ROUTES[choice], "best": ROUTES[best],
-                     "times_s": [float(x) for x in t], "estimated_mask": int(est), "regret": float(regret),
-                     "predicted_log2_us": [float(v) for v in y], "bit_identical": exact})
-        p.free()
-        print("%-28s choice %-9s best %-9s regret %.3f" % (name, ROUTES[choice], ROUTES[best], regret),
-              file=sys.stderr, flush=True)
-    tot_model = sum(r["times_s"][ROUTES.index(r["choice"])] for r in rows)
+    try:
+        for i, (name, K, M, N, blo, bhi, frac) in enumerate(mixes or HELD_OUT):
+            p = Problem(K, M, N, blo, bhi, frac, 777 + i)
+            f = p.features()
+            y = predict(cm, f[None, :])[0]
+            choice = p.chosen()
+            t, est = p.times(reps=reps)
+            t_auto = p.time_auto(reps=max(reps, 5))
+            best = int(np.argmin(t))
+            exact = None
+            if check_exact:
+                exact = bool((p.run(-1) == p.run(0)).all())
+            rows.append({"mix": name, "K": K, "M": M, "N": N, "choice": SCHEDULES[choice], "best": SCHEDULES[best],
+                         "times_s": [float(x) for x in t], "estimated_mask": int(est), "t_auto_s": t_auto,
+                         "regret": float(t_auto / t[best] - 1.0),
+                         "regret_schedule_only": float(t[choice] / t[best] - 1.0),
+                         "predicted_log2_us": [float(v) for v in y], "bit_identical": exact})
+            p.free()
+            print("%-28s choice %-13s best %-13s regret %.3f (schedule alone %.3f)" % (
+                name, SCHEDULES[choice], SCHEDULES[best], rows[-1]["regret"], rows[-1]["regret_schedule_only"]),
+                file=sys.stderr, flush=True)
+    finally:
+        L.mgx_dispatch_set_model(None)
+    tot_model = sum(r["t_auto_s"] for r in rows)
     tot_best = sum(min(r["times_s"]) for r in rows)
-    tot_fixed = {ROUTES[k]: sum(min(r["times_s"][k], 1e9) for r in rows) for k in range(O)}
+    tot_fixed = {SCHEDULES[k]: sum(min(r["times_s"][k], 1e9) for r in rows) for k in range(O)}
+    # tuning at run time: the first call of each new shape times every
+    # schedule once (warm-up + one run each) before running the best
+    tot_tuning = sum(2 * sum(x for x in r["times_s"] if x < 1e29) + min(r["times_s"]) for r in rows)
     return {"mixes": rows, "max_regret": max(r["regret"] for r in rows),
             "mean_regret": float(np.mean([r["regret"] for r in rows])),
             "total_time_model_s": tot_model, "total_time_exhaustive_best_s": tot_best,
-            "total_time_fixed_route_s": tot_fixed}
+            "total_time_fixed_schedule_s": tot_fixed, "total_time_tuning_at_run_time_s": tot_tuning}
 
 
 def main():
@@ -281,7 +313,7 @@ def main():
         tr, va = idx[: int(0.9 * n)], idx[int(0.9 * n):]
         cm, mean, scale, loss = train_model(X[tr], Y[tr], mask[tr], epochs=a.epochs)
         pv = predict(cm, X[va])
-        ch = np.argmin(np.where(mask[va] > 0, pv, 1e9), 1)
+        ch = np.argmin(np.where(mask[va] > 0, pv, 1e9), 1)   # infeasible schedules are masked
         tv = T[va]
         reg = tv[np.arange(len(va)), ch] / tv.min(1) - 1
         meta = {"configs": int(n), "train_loss": float(loss), "val_mean_regret": float(reg.mean()),

@@ -48,6 +48,7 @@ struct PlainJob {
     int64_t K, M, N, Crs, Ars, Brs, W, cap;
     uint64_t *C;
     const uint64_t *A, *B;
+    int toom;   // Toom-22 cutoff of this product's schedule
 };
 
 struct Queue {
@@ -159,13 +160,13 @@ __global__ __launch_bounds__(256) void batch_recon_carry_kernel(Queue q, const R
 }
 
 template <int NB>
-__global__ __launch_bounds__(256) void batch_plain_kernel(Queue q, const PlainJob *jobs, int toom, unsigned *err)
+__global__ __launch_bounds__(256) void batch_plain_kernel(Queue q, const PlainJob *jobs, unsigned *err)
 {
     int j;
     int64_t t;
     while (next_item(q, j, t)) {
         const PlainJob &p = jobs[j];
-        plain_entries<NB, false, false>(t, p.K, p.M, p.N, p.C, p.Crs, p.A, p.Ars, p.B, p.Brs, p.W, p.cap, toom, err);
+        plain_entries<NB, false, false>(t, p.K, p.M, p.N, p.C, p.Crs, p.A, p.Ars, p.B, p.Brs, p.W, p.cap, p.toom, err);
     }
 }
 
@@ -206,9 +207,9 @@ struct Layout {
 };
 
 template <int NB>
-void launch_plain(int grid, hipStream_t s, Queue q, const PlainJob *jobs, int toom, unsigned *err)
+void launch_plain(int grid, hipStream_t s, Queue q, const PlainJob *jobs, unsigned *err)
 {
-    hipLaunchKernelGGL(batch_plain_kernel<NB>, dim3(grid), dim3(256), 0, s, q, jobs, toom, err);
+    hipLaunchKernelGGL(batch_plain_kernel<NB>, dim3(grid), dim3(256), 0, s, q, jobs, err);
 }
 
 }  // namespace
@@ -224,6 +225,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
     b->routes.assign(count, MGX_ROUTE_CRT);
     int st = MGX_OK;
     std::vector<int64_t> bitsA(count), bitsB(count);
+    std::vector<int> slices(count, -1), tooms(count, (int)param(MGX_PARAM_TOOM22));
     std::vector<PlanView> views(count);
     for (int64_t i = 0; i < count && st == MGX_OK; i++) {
         const mgx_product &p = prods[i];
@@ -238,7 +240,15 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
         bitsA[i] = std::max<int64_t>(bitsA[i], 1);
         bitsB[i] = std::max<int64_t>(bitsB[i], 1);
         int r = p.route;
-        if (st == MGX_OK && r == MGX_ROUTE_AUTO) st = mgx_dispatch_choose(p.K, p.M, p.N, p.A, p.Ars, p.B, p.Brs, p.cap, &r);
+        if (st == MGX_OK && r == MGX_ROUTE_AUTO) {
+            int sci = 0;
+            st = mgx_dispatch_choose_schedule(p.K, p.M, p.N, p.A, p.Ars, p.B, p.Brs, p.cap, &sci);
+            mgx_schedule sc{};
+            if (st == MGX_OK) st = mgx_schedule_get(sci, &sc);
+            r = sc.route;
+            slices[i] = sc.slices;   // the batch's residue GEMM has one tile shape
+            tooms[i] = sc.toom;
+        }
         // the batch runs plain products as one flat per-entry launch
         if (r == MGX_ROUTE_WINOGRAD1 || r == MGX_ROUTE_WINOGRAD2) r = MGX_ROUTE_PLAIN;
         if (r == MGX_ROUTE_PLAIN && (std::max(bitsA[i], bitsB[i]) + 63) / 64 * 2 > 64 * 8)
@@ -246,7 +256,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
         b->routes[i] = r;
         if (st == MGX_OK && r == MGX_ROUTE_CRT) {
             const mgx_crt_plan *pl = nullptr;
-            st = cached_plan(p.M, bitsA[i], bitsB[i], &pl);
+            st = cached_plan(p.M, bitsA[i], bitsB[i], &pl, slices[i]);
             if (st == MGX_OK) {
                 b->plans.push_back(pl);   // the graph reads its tables on every replay
                 st = plan_view(pl, &views[i]);
@@ -370,6 +380,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             PlainJob q;
             q.K = p.K; q.M = p.M; q.N = p.N; q.Crs = p.Crs; q.Ars = p.Ars; q.Brs = p.Brs; q.W = p.cap + 1; q.cap = p.cap;
             q.C = (uint64_t *)p.C; q.A = (const uint64_t *)p.A; q.B = (const uint64_t *)p.B;
+            q.toom = tooms[i];
             pj[pclass[i]].push_back(q);
             pp[pclass[i]].push_back(pp[pclass[i]].back() + ceil_div(p.K * p.N, 4));
         }
@@ -404,7 +415,6 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
     unsigned long long *ctr = (unsigned long long *)(base + o_ctr);
     b->counters = ctr;
     unsigned *err = device_error_word();
-    const int toom = (int)param(MGX_PARAM_TOOM22);
     auto queue = [&](int slot, int64_t o_prefix, size_t njobs) {
         Queue q;
         q.counter = ctr + slot;
@@ -424,12 +434,12 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             b->queues.push_back({MGX_QUEUE_PLAIN, 9 + c, pp[c].back(), grid});
             const PlainJob *jobs = (const PlainJob *)(base + o_pj[c]);
             switch (NB_CLASSES[c]) {
-            case 1: launch_plain<1>(grid, cs, q, jobs, toom, err); break;
-            case 2: launch_plain<2>(grid, cs, q, jobs, toom, err); break;
-            case 3: launch_plain<3>(grid, cs, q, jobs, toom, err); break;
-            case 4: launch_plain<4>(grid, cs, q, jobs, toom, err); break;
-            case 6: launch_plain<6>(grid, cs, q, jobs, toom, err); break;
-            default: launch_plain<8>(grid, cs, q, jobs, toom, err); break;
+            case 1: launch_plain<1>(grid, cs, q, jobs, err); break;
+            case 2: launch_plain<2>(grid, cs, q, jobs, err); break;
+            case 3: launch_plain<3>(grid, cs, q, jobs, err); break;
+            case 4: launch_plain<4>(grid, cs, q, jobs, err); break;
+            case 6: launch_plain<6>(grid, cs, q, jobs, err); break;
+            default: launch_plain<8>(grid, cs, q, jobs, err); break;
             }
         }
         for (int c = 0; c < 6; c++) {

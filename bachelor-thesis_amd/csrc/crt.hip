@@ -325,6 +325,16 @@ extern "C" {
 
 int mgx_crt_plan_create(mgx_crt_plan **out, int64_t K, int64_t bitsA, int64_t bitsB)
 {
+    return plan_create_slices(out, K, bitsA, bitsB, param(MGX_PARAM_CRT_SLICES));
+}
+
+}  // extern "C"
+
+namespace mgx {
+// A plan with the digit count given (0: 2 when the primes below 65280
+// suffice, else 4; 2 or 4 to force).
+int plan_create_slices(mgx_crt_plan **out, int64_t K, int64_t bitsA, int64_t bitsB, int64_t want)
+{
     if (!out || K < 1 || bitsA < 0 || bitsB < 0) return fail(MGX_ERR_ARG, "bad plan arguments");
     MGX_TRY(ensure_init());
     if (bitsA == 0) bitsA = 1;
@@ -335,7 +345,6 @@ int mgx_crt_plan_create(mgx_crt_plan **out, int64_t K, int64_t bitsA, int64_t bi
     pl->bitsB = bitsB;
     pl->bound = bitsA + bitsB + clog2(K);
     hb::Nat M;
-    const int64_t want = param(MGX_PARAM_CRT_SLICES);
     if (want != 0 && want != 2 && want != 4) {
         delete pl;
         return fail(MGX_ERR_ARG, "slice count %lld is not 0, 2 or 4", (long long)want);
@@ -424,6 +433,9 @@ int mgx_crt_plan_create(mgx_crt_plan **out, int64_t K, int64_t bitsA, int64_t bi
     *out = pl;
     return MGX_OK;
 }
+}  // namespace mgx
+
+extern "C" {
 
 int mgx_crt_plan_destroy(mgx_crt_plan *pl)
 {
@@ -485,18 +497,31 @@ int mgx_crt_reduce(const mgx_crt_plan *pl, int transpose, int64_t rows, int64_t 
 int mgx_crt_residue_gemm(const mgx_crt_plan *pl, int64_t K, int64_t M, int64_t N, const int8_t *RA,
                          const int8_t *RB, uint32_t *Y, void *stream)
 {
+    return residue_gemm_launch(pl, K, M, N, RA, RB, Y, as_stream(stream), (int)param(MGX_PARAM_GEMM_ROWS));
+}
+
+}  // extern "C"
+
+namespace mgx {
+// rows: the S = 2 tile's height, 128 (also for 0) or 64 (more workgroups for
+// small products); S = 4 products have one tile shape
+int residue_gemm_launch(const mgx_crt_plan *pl, int64_t K, int64_t M, int64_t N, const int8_t *RA, const int8_t *RB,
+                        uint32_t *Y, hipStream_t stream, int rows)
+{
     if (!pl || !RA || !RB || !Y) return fail(MGX_ERR_ARG, "null argument");
     int64_t rpadA, rpadB, kpad;
     mgx_crt_residue_dims(pl, K, M, &rpadA, &kpad);
     mgx_crt_residue_dims(pl, N, M, &rpadB, nullptr);
     int64_t ystride = crt_ystride(K * N);
-    if (pl->S == 2) {   // 128 x 128 tiles of prime pairs
-        int ntm = (int)(rpadA / 128), ntn = (int)(rpadB / 128);
+    if (pl->S == 2) {   // rows x 128 tiles of prime pairs (rpadA is a multiple of 128)
+        const int wm = rows == 64 ? 1 : 2;
+        int ntm = (int)(rpadA / (64 * wm)), ntn = (int)(rpadB / 128);
         int64_t total = (int64_t)ntm * ntn * ceil_div(pl->P, 2);
         int64_t grid = round_up(total, 8);
         if (grid > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "product too large for one GEMM launch");
-        auto kern = kpad > KCHUNK2 ? crt_gemm2_kernel<true, 2, 2> : crt_gemm2_kernel<false, 2, 2>;
-        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, as_stream(stream), pl->d_pc, (int)pl->P, RA,
+        auto kern = wm == 1 ? (kpad > KCHUNK2 ? crt_gemm2_kernel<true, 1, 2> : crt_gemm2_kernel<false, 1, 2>)
+                            : (kpad > KCHUNK2 ? crt_gemm2_kernel<true, 2, 2> : crt_gemm2_kernel<false, 2, 2>);
+        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(128 * wm), 0, stream, pl->d_pc, (int)pl->P, RA,
                            RB, rpadA * kpad, rpadB * kpad, kpad, K, N, ntm, ntn, total, Y, ystride);
         MGX_HIP(hipGetLastError());
         return MGX_OK;
@@ -507,16 +532,15 @@ int mgx_crt_residue_gemm(const mgx_crt_plan *pl, int64_t K, int64_t M, int64_t N
     if (grid > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "product too large for one GEMM launch");
     // products with K beyond one chunk take the variant that folds between chunks
     if (kpad <= KCHUNK)
-        hipLaunchKernelGGL(crt_gemm_kernel<false>, dim3((unsigned)grid), dim3(256), 0, as_stream(stream), pl->d_pc,
+        hipLaunchKernelGGL(crt_gemm_kernel<false>, dim3((unsigned)grid), dim3(256), 0, stream, pl->d_pc,
                            RA, RB, rpadA * kpad, rpadB * kpad, kpad, K, N, ntm, ntn, total, Y, ystride);
     else
-        hipLaunchKernelGGL(crt_gemm_kernel<true>, dim3((unsigned)grid), dim3(256), 0, as_stream(stream), pl->d_pc,
+        hipLaunchKernelGGL(crt_gemm_kernel<true>, dim3((unsigned)grid), dim3(256), 0, stream, pl->d_pc,
                            RA, RB, rpadA * kpad, rpadB * kpad, kpad, K, N, ntm, ntn, total, Y, ystride);
     MGX_HIP(hipGetLastError());
     return MGX_OK;
 }
-
-}  // extern "C"
+}  // namespace mgx
 
 namespace {
 int recon_row_blocks(const mgx_crt_plan *pl) { return (int)ceil_div(4 * pl->nw, RECON_ROWS); }
@@ -648,7 +672,7 @@ int mgx_crt_mfma_matmul(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t 
     } else if (plan->K < M) {
         return fail(MGX_ERR_SHAPE, "plan made for inner dimension %lld < %lld", (long long)plan->K, (long long)M);
     }
-    int st = crt_matmul_with(pl, K, M, N, C, Crs, A, Ars, B, Brs, cap, s);
+    int st = crt_matmul_with(pl, K, M, N, C, Crs, A, Ars, B, Brs, cap, s, (int)param(MGX_PARAM_GEMM_ROWS));
     if (!plan) release_plan(pl);
     return st;
 }
@@ -683,7 +707,8 @@ CrtLayout crt_layout(const mgx_crt_plan *pl, int64_t K, int64_t M, int64_t N)
 
 // The stage launches of one product on s (no allocation, no synchronization).
 int crt_launch(const mgx_crt_plan *pl, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A,
-               int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap, char *scr, const CrtLayout &L, hipStream_t s)
+               int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap, char *scr, const CrtLayout &L, hipStream_t s,
+               int gemm_rows)
 {
     void *stream = s;
     int8_t *RA = (int8_t *)scr, *RB = (int8_t *)scr + L.bytesA;
@@ -703,7 +728,7 @@ int crt_launch(const mgx_crt_plan *pl, int64_t K, int64_t M, int64_t N, mgx_limb
         MGX_TRY(mgx_crt_reduce(pl, 0, K, M, A, Ars, cap, 0, RA, stream));
         MGX_TRY(mgx_crt_reduce(pl, 1, N, M, B, Brs, cap, 0, RB, stream));
     }
-    MGX_TRY(mgx_crt_residue_gemm(pl, K, M, N, RA, RB, Y, stream));
+    MGX_TRY(residue_gemm_launch(pl, K, M, N, RA, RB, Y, s, gemm_rows));
     return reconstruct_with(pl, K, N, Y, C, Crs, cap, scr + L.offR, s);
 }
 }  // namespace
@@ -711,13 +736,14 @@ int crt_launch(const mgx_crt_plan *pl, int64_t K, int64_t M, int64_t N, mgx_limb
 // The product through a plan (stream-ordered), then one synchronization for
 // the device error word.
 int crt_matmul_with(const mgx_crt_plan *pl, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs,
-                    const mgx_limb *A, int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap, hipStream_t s)
+                    const mgx_limb *A, int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap, hipStream_t s,
+                    int gemm_rows)
 {
     const CrtLayout L = crt_layout(pl, K, M, N);
     char *scr = nullptr;
     int st = scratch_alloc((void **)&scr, L.total, s);
     if (st == MGX_OK) {
-        st = crt_launch(pl, K, M, N, C, Crs, A, Ars, B, Brs, cap, scr, L, s);
+        st = crt_launch(pl, K, M, N, C, Crs, A, Ars, B, Brs, cap, scr, L, s, gemm_rows);
         scratch_free(scr, s);
     }
     if (st == MGX_OK) st = check_device_error(s);
@@ -793,11 +819,12 @@ std::vector<CachedPlan> g_plans;
 uint64_t g_plan_clock = 0;
 }  // namespace
 
-int cached_plan(int64_t K, int64_t bitsA, int64_t bitsB, const mgx_crt_plan **out)
+int cached_plan(int64_t K, int64_t bitsA, int64_t bitsB, const mgx_crt_plan **out, int slices)
 {
     int dev = 0;
     MGX_HIP(hipGetDevice(&dev));
-    std::array<int64_t, 5> key = {dev, clog2(K), bitsA, bitsB, param(MGX_PARAM_CRT_SLICES)};
+    const int64_t want = slices >= 0 ? slices : param(MGX_PARAM_CRT_SLICES);
+    std::array<int64_t, 5> key = {dev, clog2(K), bitsA, bitsB, want};
     std::lock_guard<std::mutex> lk(g_plan_mu);
     for (auto &c : g_plans)
         if (c.key == key) {
@@ -807,7 +834,7 @@ int cached_plan(int64_t K, int64_t bitsA, int64_t bitsB, const mgx_crt_plan **ou
             return MGX_OK;
         }
     mgx_crt_plan *pl = nullptr;
-    MGX_TRY(mgx_crt_plan_create(&pl, (int64_t)1 << clog2(K), bitsA, bitsB));
+    MGX_TRY(plan_create_slices(&pl, (int64_t)1 << clog2(K), bitsA, bitsB, want));
     size_t unpinned = 0;
     for (auto &c : g_plans) unpinned += c.pins == 0;
     if (unpinned >= PLAN_CACHE_MAX) {
@@ -880,7 +907,7 @@ int mgx_crt_mfma_matmul_noalloc(const mgx_crt_plan *plan, int64_t K, int64_t M, 
     if (L.total > NOALLOC_SCRATCH_BYTES)
         return fail(MGX_ERR_SHAPE, "this product needs %zu bytes of scratch; the no-allocation entry point has %zu",
                     L.total, NOALLOC_SCRATCH_BYTES);
-    return crt_launch(plan, K, M, N, C, Crs, A, Ars, B, Brs, cap, scr, L, s);
+    return crt_launch(plan, K, M, N, C, Crs, A, Ars, B, Brs, cap, scr, L, s, (int)param(MGX_PARAM_GEMM_ROWS));
 }
 
 int64_t mgx_crt_noalloc_scratch_bytes(const mgx_crt_plan *plan, int64_t K, int64_t M, int64_t N)

@@ -283,6 +283,7 @@ int mgx_costmodel_destroy(mgx_costmodel *cm)
 }
 
 int64_t mgx_costmodel_param_count(const mgx_costmodel *cm) { return cm ? cm->np : -1; }
+int64_t mgx_costmodel_outputs(const mgx_costmodel *cm) { return cm ? cm->O : -1; }
 
 int mgx_costmodel_get_params(const mgx_costmodel *cm, float *out)
 {

@@ -123,7 +123,7 @@ void launch_mul_nb(dim3 g, hipStream_t s, int64_t K, int64_t M, int64_t N, uint6
 
 template <bool S, bool AM>
 int launch_mul(int64_t K, int64_t M, int64_t N, uint64_t *C, int64_t Crs, const uint64_t *A, int64_t Ars,
-               const uint64_t *B, int64_t Brs, int64_t cap, int64_t opwords, hipStream_t s)
+               const uint64_t *B, int64_t Brs, int64_t cap, int64_t opwords, hipStream_t s, int toom)
 {
     MGX_TRY(ensure_init());
     if (K < 0 || M < 0 || N < 0 || cap < 1) return fail(MGX_ERR_SHAPE, "bad shape");
@@ -135,7 +135,7 @@ int launch_mul(int64_t K, int64_t M, int64_t N, uint64_t *C, int64_t Crs, const 
     int64_t nbneed = std::min<int64_t>((opwords + 63) / 64, 8);
     dim3 g((unsigned)ceil_div(K * N, 4));
     unsigned *err = device_error_word();
-    int toom = (int)param(MGX_PARAM_TOOM22);
+    if (toom < 0) toom = (int)param(MGX_PARAM_TOOM22);
     const int64_t W = cap + 1;
 #define MGX_CASE(nb) \
     if (nbneed <= nb) { launch_mul_nb<nb, S, AM>(g, s, K, M, N, C, Crs, A, Ars, B, Brs, W, cap, toom, err); MGX_HIP(hipGetLastError()); return MGX_OK; }
@@ -164,12 +164,12 @@ int launch_aors(int64_t Wd, int64_t H, uint64_t *C, int64_t Crs, const uint64_t 
 namespace mgx {
 // operand-size-aware entry used by the recursion drivers
 int plain_mul(bool strided, bool addmul, int64_t K, int64_t M, int64_t N, uint64_t *C, int64_t Crs, const uint64_t *A,
-              int64_t Ars, const uint64_t *B, int64_t Brs, int64_t cap, int64_t opwords, hipStream_t s)
+              int64_t Ars, const uint64_t *B, int64_t Brs, int64_t cap, int64_t opwords, hipStream_t s, int toom)
 {
-    if (strided) return addmul ? launch_mul<true, true>(K, M, N, C, Crs, A, Ars, B, Brs, cap, opwords, s)
-                               : launch_mul<true, false>(K, M, N, C, Crs, A, Ars, B, Brs, cap, opwords, s);
-    return addmul ? launch_mul<false, true>(K, M, N, C, Crs, A, Ars, B, Brs, cap, opwords, s)
-                  : launch_mul<false, false>(K, M, N, C, Crs, A, Ars, B, Brs, cap, opwords, s);
+    if (strided) return addmul ? launch_mul<true, true>(K, M, N, C, Crs, A, Ars, B, Brs, cap, opwords, s, toom)
+                               : launch_mul<true, false>(K, M, N, C, Crs, A, Ars, B, Brs, cap, opwords, s, toom);
+    return addmul ? launch_mul<false, true>(K, M, N, C, Crs, A, Ars, B, Brs, cap, opwords, s, toom)
+                  : launch_mul<false, false>(K, M, N, C, Crs, A, Ars, B, Brs, cap, opwords, s, toom);
 }
 int plain_aors(bool strided, bool sub, int64_t Wd, int64_t H, uint64_t *C, int64_t Crs, const uint64_t *A, int64_t Ars,
                const uint64_t *B, int64_t Brs, int64_t cap, hipStream_t s)

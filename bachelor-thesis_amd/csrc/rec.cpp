@@ -12,12 +12,6 @@
 #include <chrono>
 #include <algorithm>
 
-namespace mgx {
-int plain_mul(bool strided, bool addmul, int64_t K, int64_t M, int64_t N, uint64_t *C, int64_t Crs, const uint64_t *A,
-              int64_t Ars, const uint64_t *B, int64_t Brs, int64_t cap, int64_t opwords, hipStream_t s);
-int plain_aors(bool strided, bool sub, int64_t Wd, int64_t H, uint64_t *C, int64_t Crs, const uint64_t *A, int64_t Ars,
-               const uint64_t *B, int64_t Brs, int64_t cap, hipStream_t s);
-}
 
 extern "C" int mgx_crt_scan_bits(int64_t rows, int64_t cols, const mgx_limb *X, int64_t Xrs, int64_t cap, int64_t *bits,
                                  void *stream);
@@ -74,6 +68,7 @@ struct Ctx {
     bool strided;
     int64_t leaf;
     int64_t maxdepth;
+    int toom;   // Toom-22 block cutoff for the leaf products (< 0: MGX_PARAM_TOOM22)
 };
 
 bool can_split(const Ctx &c, int64_t N, int depth)
@@ -89,7 +84,7 @@ int rec(const Ctx &c, int method, bool addmul, int64_t N, uint64_t *C, int64_t C
         const uint64_t *B, int64_t Brs, int64_t opbits, int depth)
 {
     if (!can_split(c, N, depth))
-        return plain_mul(c.strided, addmul, N, N, N, C, Crs, A, Ars, B, Brs, c.cap, (opbits + 31) / 32, c.s);
+        return plain_mul(c.strided, addmul, N, N, N, C, Crs, A, Ars, B, Brs, c.cap, (opbits + 31) / 32, c.s, c.toom);
     const int64_t h = N / 2, W = c.cap + 1;
     const Step *seq;
     size_t ns;
@@ -155,7 +150,7 @@ int squarify(const Ctx &c, int method, int64_t K, int64_t M, int64_t N, uint64_t
 {
     if (K == 0 || M == 0 || N == 0) return MGX_OK;
     if (K <= c.leaf || M <= c.leaf || N <= c.leaf)
-        return plain_mul(false, false, K, M, N, C, Crs, A, Ars, B, Brs, c.cap, (opbits + 31) / 32, c.s);
+        return plain_mul(false, false, K, M, N, C, Crs, A, Ars, B, Brs, c.cap, (opbits + 31) / 32, c.s, c.toom);
     if (K == M && M == N && odd_part(K) <= c.leaf) return rec(c, method, false, K, C, Crs, A, Ars, B, Brs, opbits, 0);
     int64_t mind = std::min(std::min(K, M), N), lt = 1;
     while (lt * 2 <= mind) lt *= 2;
@@ -215,6 +210,7 @@ Ctx make_ctx(hipStream_t s, int64_t cap, bool strided)
     c.leaf = std::max<int64_t>(param(MGX_PARAM_LEAF), 1);
     if (strided) c.leaf = std::max<int64_t>(c.leaf, MGX_INTERLEAVE_BLOCK);
     c.maxdepth = param(MGX_PARAM_WINOGRAD_DEPTH);
+    c.toom = -1;
     return c;
 }
 
@@ -239,11 +235,12 @@ int squarify_nosync(int method, int64_t K, int64_t M, int64_t N, uint64_t *C, in
 // tuner; the global tunables are left alone, so concurrent calls do not race)
 int squarify_with(int method, int64_t K, int64_t M, int64_t N, uint64_t *C, int64_t Crs, const uint64_t *A,
                   int64_t Ars, const uint64_t *B, int64_t Brs, int64_t cap, int64_t opbits, int64_t leaf,
-                  int64_t maxdepth, hipStream_t s)
+                  int64_t maxdepth, hipStream_t s, int toom)
 {
     Ctx c = make_ctx(s, cap, false);
     c.leaf = std::max<int64_t>(leaf, 1);
     c.maxdepth = maxdepth;
+    c.toom = toom;
     return squarify(c, method, K, M, N, C, Crs, A, Ars, B, Brs, opbits);
 }
 }  // namespace mgx

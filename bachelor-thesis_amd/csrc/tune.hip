@@ -11,11 +11,6 @@
 
 namespace mgx {
 int costmodel_predict1(mgx_costmodel *cm, const float *x, float *y, hipStream_t s);
-int plain_mul(bool strided, bool addmul, int64_t K, int64_t M, int64_t N, uint64_t *C, int64_t Crs, const uint64_t *A,
-              int64_t Ars, const uint64_t *B, int64_t Brs, int64_t cap, int64_t opwords, hipStream_t s);
-int squarify_with(int method, int64_t K, int64_t M, int64_t N, uint64_t *C, int64_t Crs, const uint64_t *A,
-                  int64_t Ars, const uint64_t *B, int64_t Brs, int64_t cap, int64_t opbits, int64_t leaf,
-                  int64_t maxdepth, hipStream_t s);
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z)
 {
@@ -133,12 +128,55 @@ using namespace mgx;
 
 namespace {
 
-int plain_route(int route, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A, int64_t Ars,
-                const mgx_limb *B, int64_t Brs, int64_t cap, int64_t opbits, hipStream_t s)
+// The tuner's schedule space: route, digit count and residue-GEMM tile of
+// the multi-modular route, Toom-22 cutoff and Winograd depth of the plain
+// routes.  Values -1 (slices, toom) and 0 (gemm_rows) take the global
+// tunables.
+const mgx_schedule SCHEDULES[MGX_SCHEDULES] = {
+    {MGX_ROUTE_CRT, 0, 128, 0, 0},       // two digits while the 16-bit primes suffice, 128-row tiles
+    {MGX_ROUTE_CRT, 0, 64, 0, 0},        // the same with 64-row tiles (twice the workgroups)
+    {MGX_ROUTE_CRT, 4, 128, 0, 0},       // primes below 2^31, four digits
+    {MGX_ROUTE_PLAIN, 0, 0, 2, 0},       // per-entry kernel, Karatsuba from two 64-word blocks
+    {MGX_ROUTE_PLAIN, 0, 0, 0, 0},       // per-entry kernel, schoolbook only
+    {MGX_ROUTE_WINOGRAD1, 0, 0, 2, 1},   // one Winograd level over the per-entry kernel
+    {MGX_ROUTE_WINOGRAD2, 0, 0, 2, 2},   // two levels
+    {MGX_ROUTE_WINOGRAD1, 0, 0, 0, 1},
+    {MGX_ROUTE_WINOGRAD2, 0, 0, 0, 2},
+};
+
+// the schedule a fixed route runs (global tunables for its parameters)
+mgx_schedule route_schedule(int route)
 {
-    if (route == MGX_ROUTE_PLAIN) return plain_mul(false, false, K, M, N, C, Crs, A, Ars, B, Brs, cap, (opbits + 31) / 32, s);
-    return squarify_with(MGX_REC_WINOGRAD, K, M, N, C, Crs, A, Ars, B, Brs, cap, opbits, 2,
-                         route == MGX_ROUTE_WINOGRAD1 ? 1 : 2, s);
+    mgx_schedule sc{route, -1, 0, -1, route == MGX_ROUTE_WINOGRAD1 ? 1 : route == MGX_ROUTE_WINOGRAD2 ? 2 : 0};
+    return sc;
+}
+
+// operands of maxbits bits fit the plain kernel's widest class (512 words)
+// after the schedule's Winograd levels (two bits of growth each)
+bool schedule_feasible(const mgx_schedule &sc, int64_t maxbits)
+{
+    return sc.route == MGX_ROUTE_CRT || maxbits + 2 * sc.depth <= 512 * 32;
+}
+
+// C = A*B through a schedule, operand bit bounds known; synchronizes once
+// (device error word)
+int run_schedule(const mgx_schedule &sc, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A,
+                 int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap, int64_t ba, int64_t bb, hipStream_t s)
+{
+    if (sc.route == MGX_ROUTE_CRT) {
+        const mgx_crt_plan *pl = nullptr;
+        MGX_TRY(cached_plan(M, ba, bb, &pl, sc.slices));
+        const int rows = sc.gemm_rows > 0 ? sc.gemm_rows : (int)param(MGX_PARAM_GEMM_ROWS);
+        int st = crt_matmul_with(pl, K, M, N, C, Crs, A, Ars, B, Brs, cap, s, rows);
+        release_plan(pl);
+        return st;
+    }
+    const int64_t opbits = std::max(ba, bb);
+    if (sc.route == MGX_ROUTE_PLAIN)
+        MGX_TRY(plain_mul(false, false, K, M, N, C, Crs, A, Ars, B, Brs, cap, (opbits + 31) / 32, s, sc.toom));
+    else
+        MGX_TRY(squarify_with(MGX_REC_WINOGRAD, K, M, N, C, Crs, A, Ars, B, Brs, cap, opbits, 2, sc.depth, s, sc.toom));
+    return check_device_error(s);
 }
 
 // per matrix: bit-length histogram (6 bins), sum of bits, max bits, sum of 2048-bit blocks
@@ -192,11 +230,11 @@ int features_of(int64_t K, int64_t M, int64_t N, const Stats &a, const Stats &b,
                              (double)(a.v[8] + b.v[8]) / cnt, f);
 }
 
-int choose_route(int64_t K, int64_t M, int64_t N, const float *f, int *route, hipStream_t s);
+int choose_schedule(int64_t K, int64_t M, int64_t N, const float *f, int *schedule, hipStream_t s);
 
 mgx_costmodel *g_model = nullptr;
 std::mutex g_mu;
-std::map<std::vector<int>, int> g_tuned;   // exhaustive choices by feature bucket
+std::map<std::vector<int>, int> g_tuned;   // exhaustive schedule choices by feature bucket
 
 double now() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
 
@@ -245,82 +283,144 @@ int mgx_tune_features_device(int64_t K, int64_t M, int64_t N, const mgx_limb *A,
     return features_of(K, M, N, a, b, f);
 }
 
-int mgx_matmul_route(int route, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A,
-                     int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap, void *stream)
+int mgx_schedule_get(int index, mgx_schedule *out)
+{
+    if (index < 0 || index >= MGX_SCHEDULES || !out) return fail(MGX_ERR_ARG, "no schedule %d", index);
+    *out = SCHEDULES[index];
+    return MGX_OK;
+}
+
+// One pass over the inputs gives both the dispatch's features and the bit
+// bounds whichever schedule runs needs.
+static int matmul_sched(int index, const mgx_schedule *given, int64_t K, int64_t M, int64_t N, mgx_limb *C,
+                        int64_t Crs, const mgx_limb *A, int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap,
+                        void *stream)
 {
     MGX_TRY(ensure_init());
     if (K < 0 || M < 0 || N < 0 || cap < 1) return fail(MGX_ERR_SHAPE, "bad shape");
-    if (route != MGX_ROUTE_AUTO && (route < 0 || route >= MGX_ROUTES)) return fail(MGX_ERR_ARG, "unknown route %d", route);
     hipStream_t s = as_stream(stream);
     if (K == 0 || N == 0) return MGX_OK;
     if (M == 0) {
         MGX_HIP(hipMemset2DAsync(C, (size_t)(Crs * (cap + 1) * 8), 0, (size_t)(N * (cap + 1) * 8), (size_t)K, s));
         return check_device_error(s);
     }
-    // one pass over the inputs: the features for the dispatch and the bit
-    // bounds for whichever route runs
     Stats sa, sb;
     MGX_TRY(scan_stats_pair(K, M, N, A, Ars, B, Brs, cap, sa, sb, s));
     const int64_t ba = std::max<int64_t>((int64_t)sa.v[7], 1), bb = std::max<int64_t>((int64_t)sb.v[7], 1);
-    if (route == MGX_ROUTE_AUTO) {
-        float f[MGX_FEATURES];
-        MGX_TRY(features_of(K, M, N, sa, sb, f));
-        MGX_TRY(choose_route(K, M, N, f, &route, s));
+    mgx_schedule sc;
+    if (given) {
+        sc = *given;
+    } else {
+        if (index == MGX_ROUTE_AUTO) {
+            float f[MGX_FEATURES];
+            MGX_TRY(features_of(K, M, N, sa, sb, f));
+            MGX_TRY(choose_schedule(K, M, N, f, &index, s));
+        }
+        if (index < 0 || index >= MGX_SCHEDULES) return fail(MGX_ERR_ARG, "no schedule %d", index);
+        sc = SCHEDULES[index];
     }
-    if (route == MGX_ROUTE_CRT) {
-        const mgx_crt_plan *pl = nullptr;
-        MGX_TRY(cached_plan(M, ba, bb, &pl));
-        int st = crt_matmul_with(pl, K, M, N, C, Crs, A, Ars, B, Brs, cap, s);
-        release_plan(pl);
-        return st;
-    }
-    MGX_TRY(plain_route(route, K, M, N, C, Crs, A, Ars, B, Brs, cap, std::max(ba, bb), s));
-    return check_device_error(s);
+    if (!schedule_feasible(sc, std::max(ba, bb)))
+        return fail(MGX_ERR_UNSUPPORTED, "operands of %lld bits exceed the plain kernel's 512-word limit",
+                    (long long)std::max(ba, bb));
+    return run_schedule(sc, K, M, N, C, Crs, A, Ars, B, Brs, cap, ba, bb, s);
 }
 
-int mgx_tune_time_routes(int64_t K, int64_t M, int64_t N, mgx_limb *C, const mgx_limb *A, const mgx_limb *B,
-                         int64_t cap, int reps, double *times, unsigned *estimated)
+int mgx_matmul_schedule(int index, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A,
+                        int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap, void *stream)
+{
+    return matmul_sched(index, nullptr, K, M, N, C, Crs, A, Ars, B, Brs, cap, stream);
+}
+
+int mgx_matmul_route(int route, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A,
+                     int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap, void *stream)
+{
+    if (route == MGX_ROUTE_AUTO) return matmul_sched(MGX_ROUTE_AUTO, nullptr, K, M, N, C, Crs, A, Ars, B, Brs, cap, stream);
+    if (route < 0 || route >= MGX_ROUTES) return fail(MGX_ERR_ARG, "unknown route %d", route);
+    const mgx_schedule sc = route_schedule(route);
+    return matmul_sched(0, &sc, K, M, N, C, Crs, A, Ars, B, Brs, cap, stream);
+}
+
+int mgx_tune_time_schedules(int64_t K, int64_t M, int64_t N, mgx_limb *C, const mgx_limb *A, const mgx_limb *B,
+                            int64_t cap, int reps, double *times, unsigned *estimated)
 {
     MGX_TRY(ensure_init());
     if (reps < 1) reps = 1;
     unsigned est = 0;
-    auto run = [&](int r, int64_t rows, double *out) -> int {
-        int st0 = mgx_matmul_route(r, rows, M, N, C, N, A, M, B, N, cap, nullptr);   // warm-up (plans, pools)
-        if (st0 == MGX_ERR_UNSUPPORTED || st0 == MGX_ERR_CAPACITY) {   // route cannot take this product
+    // median wall time of reps calls after one warm-up (plans, scratch);
+    // 1e30 when the schedule cannot take the product
+    auto run = [&](int i, int64_t rows, double *out) -> int {
+        int st0 = mgx_matmul_schedule(i, rows, M, N, C, N, A, M, B, N, cap, nullptr);
+        if (st0 == MGX_ERR_UNSUPPORTED || st0 == MGX_ERR_CAPACITY) {
             *out = 1e30;
             return MGX_OK;
         }
         MGX_TRY(st0);
         std::vector<double> t;
-        for (int i = 0; i < reps; i++) {
-            double t0 = now();
-            MGX_TRY(mgx_matmul_route(r, rows, M, N, C, N, A, M, B, N, cap, nullptr));
+        for (int r = 0; r < reps; r++) {
+            const double t0 = now();
+            MGX_TRY(mgx_matmul_schedule(i, rows, M, N, C, N, A, M, B, N, cap, nullptr));
             t.push_back(now() - t0);
         }
         std::sort(t.begin(), t.end());
         *out = t[t.size() / 2];
         return MGX_OK;
     };
-    MGX_TRY(run(MGX_ROUTE_CRT, K, &times[MGX_ROUTE_CRT]));
-    // one-row-block estimate of the plain route; plain routes that cannot
-    // come within 2x of the CRT time even with two Winograd levels are not run
-    int64_t rows = std::min<int64_t>(K, 8);
-    double t_sub;
-    MGX_TRY(run(MGX_ROUTE_PLAIN, rows, &t_sub));
-    double naive_est = t_sub * (double)K / (double)rows;
-    bool hopeless = naive_est * 0.7 > 2.0 * times[MGX_ROUTE_CRT] && K > rows;
-    for (int r = MGX_ROUTE_PLAIN; r < MGX_ROUTES; r++) {
-        if (hopeless) {
-            double f = r == MGX_ROUTE_PLAIN ? 1.0 : r == MGX_ROUTE_WINOGRAD1 ? 0.875 : 0.765625;
-            times[r] = naive_est * f;
-            est |= 1u << r;
-        } else if (r == MGX_ROUTE_PLAIN && rows == K) {
-            times[r] = t_sub;
+    double best_crt = 1e30;
+    for (int i = 0; i < MGX_SCHEDULES; i++)
+        if (SCHEDULES[i].route == MGX_ROUTE_CRT) {
+            MGX_TRY(run(i, K, &times[i]));
+            best_crt = std::min(best_crt, times[i]);
+        }
+    // The flat kernel's time on one block of rows estimates the plain
+    // schedules; when even two Winograd levels could not come within 2x of
+    // the best multi-modular schedule they are not run but extrapolated
+    // (flagged in *estimated): the flat time scaled by the rows, times 7/8
+    // per Winograd level.
+    const int64_t rows = std::min<int64_t>(K, 8);
+    double flat[2] = {1e30, 1e30};   // toom on, off
+    int flat_idx[2] = {-1, -1};
+    for (int i = 0; i < MGX_SCHEDULES; i++)
+        if (SCHEDULES[i].route == MGX_ROUTE_PLAIN) {
+            const int k = SCHEDULES[i].toom ? 0 : 1;
+            flat_idx[k] = i;
+            MGX_TRY(run(i, rows, &flat[k]));
+        }
+    const double scale = (double)K / (double)rows;
+    const bool hopeless = K > rows && std::min(flat[0], flat[1]) * scale * 0.7 > 2.0 * best_crt;
+    for (int i = 0; i < MGX_SCHEDULES; i++) {
+        const mgx_schedule &sc = SCHEDULES[i];
+        if (sc.route == MGX_ROUTE_CRT) continue;
+        const double base = flat[sc.toom ? 0 : 1];
+        if (base >= 1e29) {
+            times[i] = 1e30;   // the plain kernel cannot take these operands
+        } else if (hopeless) {
+            times[i] = base * scale * std::pow(0.875, sc.depth);
+            est |= 1u << i;
+        } else if (sc.route == MGX_ROUTE_PLAIN && rows == K) {
+            times[i] = base;
         } else {
-            MGX_TRY(run(r, K, &times[r]));
+            MGX_TRY(run(i, K, &times[i]));
         }
     }
     if (estimated) *estimated = est;
+    return MGX_OK;
+}
+
+int mgx_tune_time_routes(int64_t K, int64_t M, int64_t N, mgx_limb *C, const mgx_limb *A, const mgx_limb *B,
+                         int64_t cap, int reps, double *times, unsigned *estimated)
+{
+    double t[MGX_SCHEDULES];
+    unsigned est = 0;
+    MGX_TRY(mgx_tune_time_schedules(K, M, N, C, A, B, cap, reps, t, &est));
+    // each route at its default schedule: CRT two digits 128-row tiles,
+    // plain and Winograd with the Karatsuba cutoff of two blocks
+    const int pick[MGX_ROUTES] = {0, 3, 5, 6};
+    unsigned e = 0;
+    for (int r = 0; r < MGX_ROUTES; r++) {
+        times[r] = t[pick[r]];
+        if (est & (1u << pick[r])) e |= 1u << r;
+    }
+    if (estimated) *estimated = e;
     return MGX_OK;
 }
 
@@ -331,40 +431,55 @@ int mgx_dispatch_set_model(mgx_costmodel *cm)
     return MGX_OK;
 }
 
-int mgx_dispatch_choose(int64_t K, int64_t M, int64_t N, const mgx_limb *A, int64_t Ars, const mgx_limb *B,
-                        int64_t Brs, int64_t cap, int *route)
+int mgx_dispatch_choose_schedule(int64_t K, int64_t M, int64_t N, const mgx_limb *A, int64_t Ars, const mgx_limb *B,
+                                 int64_t Brs, int64_t cap, int *schedule)
 {
     float f[MGX_FEATURES];
     MGX_TRY(mgx_tune_features_device(K, M, N, A, Ars, B, Brs, cap, f));
-    return choose_route(K, M, N, f, route, nullptr);
+    return choose_schedule(K, M, N, f, schedule, nullptr);
+}
+
+int mgx_dispatch_choose(int64_t K, int64_t M, int64_t N, const mgx_limb *A, int64_t Ars, const mgx_limb *B,
+                        int64_t Brs, int64_t cap, int *route)
+{
+    int sc = 0;
+    MGX_TRY(mgx_dispatch_choose_schedule(K, M, N, A, Ars, B, Brs, cap, &sc));
+    *route = SCHEDULES[sc].route;
+    return MGX_OK;
 }
 
 }  // extern "C"
 
 namespace {
-// The route for a product's features: the cost model's prediction when a
-// model is installed, else the exhaustive autotuner's remembered choice for
-// the feature bucket.
-int choose_route(int64_t K, int64_t M, int64_t N, const float *f, int *route, hipStream_t s_pred)
+// The schedule for a product's features: the cost model's prediction when a
+// model is installed (one output per schedule; a model with one output per
+// route picks among the routes' default schedules), else the exhaustive
+// autotuner's remembered choice for the feature bucket.  Schedules that
+// cannot take the operands are never chosen.
+int choose_schedule(int64_t K, int64_t M, int64_t N, const float *f, int *schedule, hipStream_t s_pred)
 {
     mgx_costmodel *cm;
     {
         std::lock_guard<std::mutex> lk(g_mu);
         cm = g_model;
     }
+    const int64_t maxbits = (int64_t)std::llround(std::exp2((double)f[5]));
     if (cm) {
-        float y[MGX_ROUTES];
+        const int O = (int)mgx_costmodel_outputs(cm);
+        float y[64];
+        if (O != MGX_SCHEDULES && O != MGX_ROUTES) return fail(MGX_ERR_ARG, "cost model has %d outputs", O);
         MGX_TRY(costmodel_predict1(cm, f, y, s_pred));
-        // the plain kernel takes operands of at most 512 32-bit words, and
-        // each Winograd level adds up to two bits to them: routes that
-        // cannot take the product are never chosen, whatever the prediction
-        const double maxbits = std::exp2((double)f[5]);
-        int best = MGX_ROUTE_CRT;
-        for (int r = 1; r < MGX_ROUTES; r++) {
-            const int levels = r == MGX_ROUTE_WINOGRAD1 ? 1 : r == MGX_ROUTE_WINOGRAD2 ? 2 : 0;
-            if (maxbits + 2 * levels <= 512 * 32 && y[r] < y[best]) best = r;
+        const int route_pick[MGX_ROUTES] = {0, 3, 5, 6};
+        int best = 0;
+        float by = 1e30f;
+        for (int o = 0; o < O; o++) {
+            const int sc = O == MGX_SCHEDULES ? o : route_pick[o];
+            if (schedule_feasible(SCHEDULES[sc], maxbits) && y[o] < by) {
+                by = y[o];
+                best = sc;
+            }
         }
-        *route = best;
+        *schedule = best;
         return MGX_OK;
     }
     // fallback: the exhaustive autotuner's choice for this feature bucket,
@@ -374,30 +489,29 @@ int choose_route(int64_t K, int64_t M, int64_t N, const float *f, int *route, hi
     {
         std::lock_guard<std::mutex> lk(g_mu);
         auto it = g_tuned.find(key);
-        if (it != g_tuned.end()) { *route = it->second; return MGX_OK; }
+        if (it != g_tuned.end()) { *schedule = it->second; return MGX_OK; }
     }
-    int64_t maxbits = (int64_t)std::exp2(f[5]) + 1;
-    int64_t tcap = (2 * maxbits + 64 + 63) / 64 + 2;
+    const int64_t tbits = maxbits + 1;
+    const int64_t tcap = (2 * tbits + 64 + 63) / 64 + 2;
     mgx_limb *tA = nullptr, *tB = nullptr, *tC = nullptr;
     hipStream_t s = nullptr;
     int st = scratch_alloc((void **)&tA, (size_t)(K * M * (tcap + 1) * 8), s);
     if (st == MGX_OK) st = scratch_alloc((void **)&tB, (size_t)(M * N * (tcap + 1) * 8), s);
     if (st == MGX_OK) st = scratch_alloc((void **)&tC, (size_t)(K * N * (tcap + 1) * 8), s);
-    double times[MGX_ROUTES];
-    if (st == MGX_OK) st = mgx_random_matrix_device(tA, M, K, M, tcap, maxbits, maxbits, 0, 1, s);
-    if (st == MGX_OK) st = mgx_random_matrix_device(tB, N, M, N, tcap, maxbits, maxbits, 0, 2, s);
-    if (st == MGX_OK) st = mgx_tune_time_routes(K, M, N, tC, tA, tB, tcap, 1, times, nullptr);
+    double times[MGX_SCHEDULES];
+    if (st == MGX_OK) st = mgx_random_matrix_device(tA, M, K, M, tcap, tbits, tbits, 0, 1, s);
+    if (st == MGX_OK) st = mgx_random_matrix_device(tB, N, M, N, tcap, tbits, tbits, 0, 2, s);
+    if (st == MGX_OK) st = mgx_tune_time_schedules(K, M, N, tC, tA, tB, tcap, 1, times, nullptr);
     scratch_free(tA, s);
     scratch_free(tB, s);
     scratch_free(tC, s);
     MGX_TRY(st);
     int best = 0;
-    for (int r = 1; r < MGX_ROUTES; r++)
-        if (times[r] < times[best]) best = r;
+    for (int i = 1; i < MGX_SCHEDULES; i++)
+        if (times[i] < times[best]) best = i;
     std::lock_guard<std::mutex> lk(g_mu);
     g_tuned[key] = best;
-    *route = best;
+    *schedule = best;
     return MGX_OK;
 }
-
 }  // namespace

@@ -100,6 +100,12 @@ def _declare(L):
         "mgx_crt_merge_list": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_int, c_i64],
         # mgx_sched.h
         "mgx_matmul_route": [ctypes.c_int] + mul_args,
+        "mgx_matmul_schedule": [ctypes.c_int] + mul_args,
+        "mgx_schedule_get": [ctypes.c_int, c_vp],
+        "mgx_tune_time_schedules": [c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, ctypes.c_int, c_vp,
+                                    ctypes.POINTER(ctypes.c_uint)],
+        "mgx_dispatch_choose_schedule": [c_i64, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_i64,
+                                         ctypes.POINTER(ctypes.c_int)],
         "mgx_random_matrix_device": [c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_i64, ctypes.c_double,
                                      ctypes.c_uint64, c_vp],
         "mgx_tune_features": [c_i64, c_i64, c_i64, c_vp, ctypes.c_double, c_i64, ctypes.c_double, c_vp],
@@ -137,6 +143,8 @@ def _declare(L):
     L.mgx_version.restype = ctypes.c_char_p
     L.mgx_costmodel_param_count.argtypes = [c_vp]
     L.mgx_costmodel_param_count.restype = c_i64
+    L.mgx_costmodel_outputs.argtypes = [c_vp]
+    L.mgx_costmodel_outputs.restype = c_i64
     L.mgx_batch_scratch_bytes.argtypes = [c_vp]
     L.mgx_batch_scratch_bytes.restype = c_i64
     L.mgx_batch_queue_count.argtypes = [c_vp]

@@ -277,9 +277,11 @@ typedef enum {
     MGX_PARAM_WINOGRAD_DEPTH = 2, /* max recursion levels for the plain route (-1 = to the leaf) */
     MGX_PARAM_CRT_SLICES = 3,     /* CRT digits per residue for new plans: 0 = 2 when primes below 65280
                                      suffice, else 4 (default); 2 or 4 to force */
-    MGX_PARAM_TRANSFER = 4        /* mgx_matmul_host: largest block side moved to the device at once
+    MGX_PARAM_TRANSFER = 4,       /* mgx_matmul_host: largest block side moved to the device at once
                                      (reference TRANSFER_SIZE = 512, mat-gpu/mat-params.h:6); 0 = the whole
                                      product (default) */
+    MGX_PARAM_GEMM_ROWS = 5       /* residue-GEMM tile height of two-digit plans: 0 or 128 (default), or 64
+                                     (twice the workgroups, for small products) */
 } mgx_param;
 int mgx_set_param(int param, int64_t value);
 int64_t mgx_get_param(int param);

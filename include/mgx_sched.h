@@ -26,9 +26,31 @@ typedef enum {
     MGX_ROUTE_AUTO = -1       /* cost model, falling back to the autotuner */
 } mgx_route;
 
-/* C = A*B on the device through one route.  Synchronous with status. */
+/* C = A*B on the device through one route (its parameters from the global
+ * tunables), or MGX_ROUTE_AUTO: the schedule the cost model picks.
+ * Synchronous with status. */
 int mgx_matmul_route(int route, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A,
                      int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap, void *stream);
+
+/* ---------------- schedules: the autotuner's search space ---------------- */
+/* A schedule fixes what the reference fixes at compile time and sweeps by
+ * rebuilding (mat-gpu/mat-params.h:4-14, Makefile:19-22): the route, the
+ * multi-modular route's digit count (hence prime count: two digits use
+ * about twice as many primes below 65280 as four digits use below 2^31) and
+ * residue-GEMM tile, and the plain routes' Toom-22 cutoff and Winograd
+ * depth.  Every schedule is exact; only the time differs. */
+#define MGX_SCHEDULES 9
+typedef struct {
+    int route;       /* mgx_route */
+    int slices;      /* CRT: digits per residue (0: two while the primes below 65280 suffice, else four) */
+    int gemm_rows;   /* CRT with two digits: residue-GEMM tile height, 128 or 64 */
+    int toom;        /* plain routes: Toom-22 cutoff in 64-word blocks (0: schoolbook only) */
+    int depth;       /* Winograd levels (0 for the flat plain kernel and the CRT route) */
+} mgx_schedule;
+int mgx_schedule_get(int index, mgx_schedule *out);
+/* C = A*B through schedule index (or MGX_ROUTE_AUTO).  Synchronous. */
+int mgx_matmul_schedule(int index, int64_t K, int64_t M, int64_t N, mgx_limb *C, int64_t Crs, const mgx_limb *A,
+                        int64_t Ars, const mgx_limb *B, int64_t Brs, int64_t cap, void *stream);
 
 /* Device-side seeded entries (a counter-based SplitMix64 hash of seed, entry
  * and limb): each entry has bits_hi bits with probability frac_hi, else
@@ -47,10 +69,15 @@ int mgx_tune_features(int64_t K, int64_t M, int64_t N, const int64_t *hist6, dou
  * bit lengths are limb-granular, 64 |size|; synchronizes). */
 int mgx_tune_features_device(int64_t K, int64_t M, int64_t N, const mgx_limb *A, int64_t Ars, const mgx_limb *B,
                              int64_t Brs, int64_t cap, float *out);
-/* Times every route on the given device matrices (C is scratch), median of
- * reps runs after one warm-up; a plain route whose one-row estimate is
- * hopeless against the CRT time gets that scaled estimate instead of a run
- * and its flag set in *estimated (bit r).  Times in seconds. */
+/* The exhaustive autotuner: times every schedule on the given device
+ * matrices (C is scratch, rowstrides = widths), median of reps calls after
+ * one warm-up; schedules that cannot take the product get 1e30, and plain
+ * schedules that cannot come within 2x of the best multi-modular one even
+ * with two Winograd levels are extrapolated from one block of 8 rows
+ * instead of run (bit i of *estimated).  Times in seconds. */
+int mgx_tune_time_schedules(int64_t K, int64_t M, int64_t N, mgx_limb *C, const mgx_limb *A, const mgx_limb *B,
+                            int64_t cap, int reps, double *times, unsigned *estimated);
+/* The same for the four routes at their default schedules (0, 3, 5, 6). */
 int mgx_tune_time_routes(int64_t K, int64_t M, int64_t N, mgx_limb *C, const mgx_limb *A, const mgx_limb *B,
                          int64_t cap, int reps, double *times, unsigned *estimated);
 
@@ -59,6 +86,7 @@ typedef struct mgx_costmodel mgx_costmodel;
 int mgx_costmodel_create(mgx_costmodel **cm, int D, int H, int O, uint64_t seed);
 int mgx_costmodel_destroy(mgx_costmodel *cm);
 int64_t mgx_costmodel_param_count(const mgx_costmodel *cm);
+int64_t mgx_costmodel_outputs(const mgx_costmodel *cm);   /* O: MGX_SCHEDULES (or MGX_ROUTES for a route model) */
 int mgx_costmodel_get_params(const mgx_costmodel *cm, float *out);
 int mgx_costmodel_set_params(mgx_costmodel *cm, const float *in);
 int mgx_costmodel_set_normalisation(mgx_costmodel *cm, const float *mean, const float *scale);
@@ -73,7 +101,10 @@ int mgx_costmodel_load(mgx_costmodel **cm, const char *path);
 /* Installs the model used by MGX_ROUTE_AUTO (NULL: none; the autotuner's
  * exhaustive timing decides instead).  The library does not take ownership. */
 int mgx_dispatch_set_model(mgx_costmodel *cm);
-/* The route MGX_ROUTE_AUTO would take for a device-resident product. */
+/* The schedule (and its route) MGX_ROUTE_AUTO would take for a
+ * device-resident product. */
+int mgx_dispatch_choose_schedule(int64_t K, int64_t M, int64_t N, const mgx_limb *A, int64_t Ars, const mgx_limb *B,
+                                 int64_t Brs, int64_t cap, int *schedule);
 int mgx_dispatch_choose(int64_t K, int64_t M, int64_t N, const mgx_limb *A, int64_t Ars, const mgx_limb *B,
                         int64_t Brs, int64_t cap, int *route);
 

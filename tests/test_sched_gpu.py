@@ -111,6 +111,26 @@ def test_every_route_is_exact(gpu, route):
     assert to_ints(dC.to_array((K, N, cap + 1))) == matmul(A, B)
 
 
+@pytest.mark.parametrize("schedule", range(9))
+def test_every_schedule_is_exact(gpu, schedule):
+    """Each point of the tuner's schedule space (digit count, GEMM tile,
+    Toom cutoff, Winograd depth) gives the exact product; 4000-bit operands
+    are two nearly full 64-word blocks, where the Karatsuba step fires."""
+    mgx = gpu
+    L = mgx.lib()
+    sc = (ctypes.c_int * 5)()
+    mgx.check(L.mgx_schedule_get(schedule, sc))
+    for (K, M, N, bits, seed) in [(20, 24, 18, 4000, 1), (130, 9, 70, 200, 2)]:
+        cap = (2 * bits + 8 + 63) // 64 + 2
+        A, B = rand_ints(K, M, bits, seed + 10 * schedule), rand_ints(M, N, bits, seed + 10 * schedule + 5)
+        dA, dB = mgx.DeviceBuffer.from_array(to_entries(A, cap)), mgx.DeviceBuffer.from_array(to_entries(B, cap))
+        dC = mgx.DeviceBuffer(K * N * (cap + 1) * 8)
+        mgx.check(L.mgx_matmul_schedule(schedule, K, M, N, dC.ptr, N, dA.ptr, M, dB.ptr, N, cap, None))
+        C = dC.to_array((K, N, cap + 1))
+        assert to_ints(C) == matmul(A, B), (list(sc), K, M, N)
+        assert is_normalized(C)
+
+
 def test_random_matrix_device(gpu):
     mgx = gpu
     L = sched(mgx)
