@@ -349,36 +349,34 @@ int mgx_random_matrix(mgx_limb *host, int64_t W, int64_t H, int64_t rowstride, i
     return MGX_OK;
 }
 
-// mat-gpu/mpz.cu:85-104,182-204 (out_str_16 / out_file)
+// The text form of an entry (the format mat-gpu prints, mat-gpu/mpz.cu:85-104,
+// 182-204): the magnitude in lower-case hexadecimal without leading zeros,
+// preceded by '-' when the value is negative; zero is "0".  The magnitude
+// is limbs 0 .. |size|-1 (zero limbs at the top are allowed).  Returns the
+// length written (NUL excluded), or -1 when buf is too small or |size| > cap.
 int64_t mgx_entry_to_hex(char *buf, int64_t buflen, const mgx_limb *e, int64_t cap)
 {
-    int64_t size = ((const int64_t *)e)[cap];
-    int64_t n = 0;
-    auto put = [&](char c) -> bool {
-        if (n + 1 >= buflen) return false;
-        buf[n++] = c;
-        return true;
-    };
-    if (size == 0) {
-        if (!put('0')) return -1;
-        buf[n] = 0;
-        return n;
+    static const char digits[] = "0123456789abcdef";
+    const int64_t size = ((const int64_t *)e)[cap];
+    int64_t top = size < 0 ? -size : size;
+    if (top > cap || buflen < 2) return -1;
+    while (top > 0 && e[top - 1] == 0) top--;
+    if (top == 0) {
+        buf[0] = '0';
+        buf[1] = 0;
+        return 1;
     }
-    if (size < -1 || (size == -1 && e[0] != 0)) {
-        if (!put('-')) return -1;
-    }
-    if (size < 0) size = -size;
-    if (size > cap) return -1;
-    bool print = false;
-    for (int64_t i = size - 1; i >= 0; i--)
-        for (int k = 15; k >= 0; k--) {
-            char c = "0123456789abcdef"[(e[i] >> (4 * k)) & 0xf];
-            if (c != '0') print = true;
-            if (print && !put(c)) return -1;
-        }
-    if (!print && !put('0')) return -1;
-    buf[n] = 0;
-    return n;
+    // digits of the top limb, then 16 per lower limb
+    int lead = 16;
+    while (lead > 1 && ((e[top - 1] >> (4 * (lead - 1))) & 15) == 0) lead--;
+    const int64_t len = (size < 0) + lead + 16 * (top - 1);
+    if (len + 1 > buflen) return -1;
+    char *out = buf;
+    if (size < 0) *out++ = '-';
+    for (int64_t i = top - 1; i >= 0; i--)
+        for (int k = (i == top - 1 ? lead : 16) - 1; k >= 0; k--) *out++ = digits[(e[i] >> (4 * k)) & 15];
+    *out = 0;
+    return len;
 }
 
 // mat-gpu/mpz.cu:106-146 (inp_str): leading space, '-', optional "0x", hex.

@@ -139,6 +139,7 @@ struct mgx_costmodel {
     int64_t np;
     float *params = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;
     float *d_out = nullptr, *h_out = nullptr;   // one-sample prediction: device result, pinned host copy
+    float *d_norm = nullptr;   // mean[D] then scale[D] on the device (the dispatch kernel normalises there)
     int64_t step = 0;
     // feature normalisation (host): x' = (x - mean) / scale
     std::vector<float> mean, scale;
@@ -260,7 +261,7 @@ int mgx_costmodel_create(mgx_costmodel **out, int D, int H, int O, uint64_t seed
     size_t bytes = (size_t)cm->np * 4;
     if (hipMalloc(&cm->params, bytes) != hipSuccess || hipMalloc(&cm->grad, bytes) != hipSuccess ||
         hipMalloc(&cm->m, bytes) != hipSuccess || hipMalloc(&cm->v, bytes) != hipSuccess ||
-        hipMalloc(&cm->d_out, 4 * PREDICT1_MAX_O) != hipSuccess ||
+        hipMalloc(&cm->d_out, 4 * PREDICT1_MAX_O) != hipSuccess || hipMalloc(&cm->d_norm, 8 * (size_t)D) != hipSuccess ||
         hipHostMalloc(&cm->h_out, 4 * PREDICT1_MAX_O, hipHostMallocDefault) != hipSuccess) {
         mgx_costmodel_destroy(cm);
         return fail(MGX_ERR_OOM, "cost-model allocation");
@@ -269,6 +270,8 @@ int mgx_costmodel_create(mgx_costmodel **out, int D, int H, int O, uint64_t seed
     MGX_HIP(hipMemset(cm->m, 0, bytes));
     MGX_HIP(hipMemset(cm->v, 0, bytes));
     MGX_HIP(hipMemset(cm->grad, 0, bytes));
+    MGX_HIP(hipMemcpy(cm->d_norm, cm->mean.data(), 4 * (size_t)D, hipMemcpyHostToDevice));
+    MGX_HIP(hipMemcpy(cm->d_norm + D, cm->scale.data(), 4 * (size_t)D, hipMemcpyHostToDevice));
     *out = cm;
     return MGX_OK;
 }
@@ -276,7 +279,7 @@ int mgx_costmodel_create(mgx_costmodel **out, int D, int H, int O, uint64_t seed
 int mgx_costmodel_destroy(mgx_costmodel *cm)
 {
     if (!cm) return MGX_OK;
-    hipFree(cm->params); hipFree(cm->grad); hipFree(cm->m); hipFree(cm->v); hipFree(cm->d_out);
+    hipFree(cm->params); hipFree(cm->grad); hipFree(cm->m); hipFree(cm->v); hipFree(cm->d_out); hipFree(cm->d_norm);
     if (cm->h_out) hipHostFree(cm->h_out);
     delete cm;
     return MGX_OK;
@@ -304,6 +307,8 @@ int mgx_costmodel_set_normalisation(mgx_costmodel *cm, const float *mean, const 
         cm->mean[d] = mean[d];
         cm->scale[d] = scale[d] > 0 ? scale[d] : 1.f;
     }
+    MGX_HIP(hipMemcpy(cm->d_norm, cm->mean.data(), 4 * (size_t)cm->D, hipMemcpyHostToDevice));
+    MGX_HIP(hipMemcpy(cm->d_norm + cm->D, cm->scale.data(), 4 * (size_t)cm->D, hipMemcpyHostToDevice));
     return MGX_OK;
 }
 
@@ -418,6 +423,19 @@ int mgx_costmodel_predict(mgx_costmodel *cm, const float *X, int Bn, float *Y)
 }  // extern "C"
 
 namespace mgx {
+// Device pointers and shape of a model, for kernels that run its inference
+// inside a larger launch sequence.
+int costmodel_view(const mgx_costmodel *cm, const float **params, const float **norm, int *D, int *H, int *O)
+{
+    if (cm->H > PREDICT1_MAX_H || cm->D > PREDICT1_MAX_D || cm->O > PREDICT1_MAX_O) return 1;
+    *params = cm->params;
+    *norm = cm->d_norm;
+    *D = cm->D;
+    *H = cm->H;
+    *O = cm->O;
+    return 0;
+}
+
 // The dispatch's inference for one raw feature vector x (normalised here),
 // on the caller's stream: one launch, one 4 O-byte copy, one synchronization.
 int costmodel_predict1(mgx_costmodel *cm, const float *x, float *y, hipStream_t s)

@@ -11,6 +11,7 @@
 
 namespace mgx {
 int costmodel_predict1(mgx_costmodel *cm, const float *x, float *y, hipStream_t s);
+int costmodel_view(const mgx_costmodel *cm, const float **params, const float **norm, int *D, int *H, int *O);
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z)
 {
@@ -122,6 +123,80 @@ __global__ __launch_bounds__(256) void feature_scan2_kernel(StatArgs sa, int64_t
     }
 }
 
+// The dispatch's features and inference on the device, queued right after
+// the statistics scan so that one copy and one synchronization bring back
+// both the bounds and the chosen schedule.  One workgroup: thread 0 forms the
+// features exactly as mgx_tune_features does (in double, then float),
+// threads j < H evaluate hidden unit j of each layer, threads o < O the
+// outputs, and thread 0 picks the fastest predicted schedule that can take
+// the operands.
+struct DispatchTab {
+    int plain[MGX_SCHEDULES];   // schedule is a plain route (bounded by the 512-word kernel)
+    int depth[MGX_SCHEDULES];
+};
+__global__ __launch_bounds__(256) void dispatch_kernel(unsigned long long *stats, int64_t K, int64_t M, int64_t N,
+                                                       const float *__restrict__ params, const float *__restrict__ norm,
+                                                       int D, int H, int O, DispatchTab tab)
+{
+    __shared__ float x[32], h1[256], h2[256], y[64];
+    const unsigned long long *sa = stats, *sb = stats + STAT_PITCH * 9;
+    const int j = threadIdx.x;
+    if (j == 0) {
+        double hist[6], tot = 0;
+        for (int i = 0; i < 6; i++) {
+            hist[i] = (double)(sa[STAT_PITCH * i] + sb[STAT_PITCH * i]);
+            tot += hist[i];
+        }
+        if (tot <= 0) tot = 1;
+        double cnt = (double)(K * M + M * N);
+        if (cnt <= 0) cnt = 1;
+        const double mean_bits = (double)(sa[STAT_PITCH * 6] + sb[STAT_PITCH * 6]) / cnt;
+        const unsigned long long mb = max(sa[STAT_PITCH * 7], sb[STAT_PITCH * 7]);
+        const double mean_blocks = (double)(sa[STAT_PITCH * 8] + sb[STAT_PITCH * 8]) / cnt;
+        float f[MGX_FEATURES];
+        f[0] = (float)log2((double)max<int64_t>(K, 1));
+        f[1] = (float)log2((double)max<int64_t>(M, 1));
+        f[2] = (float)log2((double)max<int64_t>(N, 1));
+        f[3] = f[0] + f[1] + f[2];
+        f[4] = (float)log2(fmax(mean_bits, 1.0));
+        f[5] = (float)log2((double)max(mb, 1ull));
+        f[6] = (float)log2(fmax(mean_blocks, 1.0));
+        for (int i = 0; i < 6; i++) f[7 + i] = (float)(hist[i] / tot);
+        for (int i = 0; i < D; i++) x[i] = (f[i] - norm[i]) / norm[D + i];
+    }
+    __syncthreads();
+    const float *W1 = params, *b1 = W1 + H * D, *W2 = b1 + H, *b2 = W2 + H * H, *W3 = b2 + H, *b3 = W3 + O * H;
+    if (j < H) {
+        float a = b1[j];
+        for (int i = 0; i < D; i++) a = fmaf(W1[j * D + i], x[i], a);
+        h1[j] = a > 0.f ? a : 0.f;
+    }
+    __syncthreads();
+    if (j < H) {
+        float a = b2[j];
+        for (int i = 0; i < H; i++) a = fmaf(W2[j * H + i], h1[i], a);
+        h2[j] = a > 0.f ? a : 0.f;
+    }
+    __syncthreads();
+    if (j < O) {
+        float a = b3[j];
+        for (int i = 0; i < H; i++) a = fmaf(W3[j * H + i], h2[i], a);
+        y[j] = a;
+    }
+    __syncthreads();
+    if (j == 0) {
+        const unsigned long long maxbits = max(sa[STAT_PITCH * 7], sb[STAT_PITCH * 7]);
+        int best = 0;
+        float by = 3.0e38f;
+        for (int o = 0; o < O; o++)
+            if ((!tab.plain[o] || maxbits + 2 * (unsigned long long)tab.depth[o] <= 512 * 32) && y[o] < by) {
+                by = y[o];
+                best = o;
+            }
+        stats[STAT_PITCH * 18] = (unsigned long long)best;   // one more line after the counters
+    }
+}
+
 }  // namespace mgx
 
 using namespace mgx;
@@ -184,13 +259,16 @@ struct Stats {
     unsigned long long v[9];
 };
 
+int costmodel_view_locked(const float **params, const float **norm, int *D, int *H, int *O);
+
 // Statistics of A (K x M) and B (M x N) with one launch and one
-// synchronization.
+// synchronization; with `schedule`, also the installed cost model's choice
+// (-1 when no model with one output per schedule is installed).
 int scan_stats_pair(int64_t K, int64_t M, int64_t N, const mgx_limb *A, int64_t Ars, const mgx_limb *B, int64_t Brs,
-                    int64_t cap, Stats &sa_out, Stats &sb_out, hipStream_t s)
+                    int64_t cap, Stats &sa_out, Stats &sb_out, hipStream_t s, int *schedule = nullptr)
 {
     unsigned long long *d = nullptr;
-    const size_t bytes = 2 * 9 * STAT_PITCH * 8;
+    const size_t bytes = (2 * 9 + 1) * STAT_PITCH * 8;
     MGX_TRY(scratch_alloc((void **)&d, bytes, s));
     MGX_HIP(hipMemsetAsync(d, 0, bytes, s));
     StatArgs sa{};
@@ -209,7 +287,22 @@ int scan_stats_pair(int64_t K, int64_t M, int64_t N, const mgx_limb *A, int64_t 
     if (grid[0] + grid[1] > 0)
         hipLaunchKernelGGL(feature_scan2_kernel, dim3(grid[0] + grid[1]), dim3(256), 0, s, sa, cap + 1, d);
     MGX_HIP(hipGetLastError());
-    unsigned long long h[2 * 9 * STAT_PITCH];
+    bool inferred = false;
+    if (schedule) {
+        const float *params, *norm;
+        int D, H, O;
+        if (costmodel_view_locked(&params, &norm, &D, &H, &O) == 0 && O == MGX_SCHEDULES && D == MGX_FEATURES) {
+            DispatchTab tab;
+            for (int i = 0; i < MGX_SCHEDULES; i++) {
+                tab.plain[i] = SCHEDULES[i].route != MGX_ROUTE_CRT;
+                tab.depth[i] = SCHEDULES[i].depth;
+            }
+            hipLaunchKernelGGL(dispatch_kernel, dim3(1), dim3(256), 0, s, d, K, M, N, params, norm, D, H, O, tab);
+            MGX_HIP(hipGetLastError());
+            inferred = true;
+        }
+    }
+    unsigned long long h[(2 * 9 + 1) * STAT_PITCH];
     MGX_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, s));
     MGX_TRY(scratch_free(d, s));
     MGX_HIP(hipStreamSynchronize(s));
@@ -217,6 +310,7 @@ int scan_stats_pair(int64_t K, int64_t M, int64_t N, const mgx_limb *A, int64_t 
         sa_out.v[c] = h[STAT_PITCH * c];
         sb_out.v[c] = h[STAT_PITCH * (9 + c)];
     }
+    if (schedule) *schedule = inferred ? (int)h[STAT_PITCH * 18] : -1;
     return MGX_OK;
 }
 
@@ -235,6 +329,16 @@ int choose_schedule(int64_t K, int64_t M, int64_t N, const float *f, int *schedu
 mgx_costmodel *g_model = nullptr;
 std::mutex g_mu;
 std::map<std::vector<int>, int> g_tuned;   // exhaustive schedule choices by feature bucket
+
+// the installed model's device view, under the dispatch lock
+int costmodel_view_locked(const float **params, const float **norm, int *D, int *H, int *O)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_model) return 1;
+    return costmodel_view(g_model, params, norm, D, H, O);
+}
+
+thread_local int g_last_schedule = -1;   // the schedule this thread's last MGX_ROUTE_AUTO call ran
 
 double now() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
 
@@ -305,13 +409,17 @@ static int matmul_sched(int index, const mgx_schedule *given, int64_t K, int64_t
         return check_device_error(s);
     }
     Stats sa, sb;
-    MGX_TRY(scan_stats_pair(K, M, N, A, Ars, B, Brs, cap, sa, sb, s));
+    int inferred = -1;   // the model's choice, when it ran on the device beside the scan
+    MGX_TRY(scan_stats_pair(K, M, N, A, Ars, B, Brs, cap, sa, sb, s,
+                            !given && index == MGX_ROUTE_AUTO ? &inferred : nullptr));
     const int64_t ba = std::max<int64_t>((int64_t)sa.v[7], 1), bb = std::max<int64_t>((int64_t)sb.v[7], 1);
     mgx_schedule sc;
     if (given) {
         sc = *given;
     } else {
-        if (index == MGX_ROUTE_AUTO) {
+        if (index == MGX_ROUTE_AUTO && inferred >= 0) {
+            index = inferred;
+        } else if (index == MGX_ROUTE_AUTO) {
             float f[MGX_FEATURES];
             MGX_TRY(features_of(K, M, N, sa, sb, f));
             MGX_TRY(choose_schedule(K, M, N, f, &index, s));
@@ -322,6 +430,7 @@ static int matmul_sched(int index, const mgx_schedule *given, int64_t K, int64_t
     if (!schedule_feasible(sc, std::max(ba, bb)))
         return fail(MGX_ERR_UNSUPPORTED, "operands of %lld bits exceed the plain kernel's 512-word limit",
                     (long long)std::max(ba, bb));
+    if (!given) g_last_schedule = index;
     return run_schedule(sc, K, M, N, C, Crs, A, Ars, B, Brs, cap, ba, bb, s);
 }
 
@@ -423,6 +532,8 @@ int mgx_tune_time_routes(int64_t K, int64_t M, int64_t N, mgx_limb *C, const mgx
     if (estimated) *estimated = e;
     return MGX_OK;
 }
+
+int mgx_dispatch_last(void) { return g_last_schedule; }
 
 int mgx_dispatch_set_model(mgx_costmodel *cm)
 {

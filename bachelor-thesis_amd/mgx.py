@@ -143,6 +143,7 @@ def _declare(L):
     L.mgx_version.restype = ctypes.c_char_p
     L.mgx_costmodel_param_count.argtypes = [c_vp]
     L.mgx_costmodel_param_count.restype = c_i64
+    L.mgx_dispatch_last.argtypes = []
     L.mgx_costmodel_outputs.argtypes = [c_vp]
     L.mgx_costmodel_outputs.restype = c_i64
     L.mgx_batch_scratch_bytes.argtypes = [c_vp]

@@ -107,6 +107,11 @@ int mgx_dispatch_choose_schedule(int64_t K, int64_t M, int64_t N, const mgx_limb
                                  int64_t Brs, int64_t cap, int *schedule);
 int mgx_dispatch_choose(int64_t K, int64_t M, int64_t N, const mgx_limb *A, int64_t Ars, const mgx_limb *B,
                         int64_t Brs, int64_t cap, int *route);
+/* The schedule the calling thread's last MGX_ROUTE_AUTO product ran (-1:
+ * none yet).  With a model installed, that product's features and inference
+ * ran on the device right after its statistics scan (one launch, no extra
+ * synchronization). */
+int mgx_dispatch_last(void);
 
 /* ---------------- measured ceilings (no reference counterpart) ---------------- */
 /* Measures, on the current device, the INT8 MFMA issue rate

@@ -101,6 +101,33 @@ def test_hex_roundtrip_matches_reference_format():
     assert L.mgx_entry_from_hex(bad.ctypes.data, cap, b"xyz") == 7
 
 
+def test_hex_against_python_format():
+    """Python's own hexadecimal formatting is the format's definition: random
+    magnitudes, zero limbs inside, a zero top limb left in the size (not
+    normalized), and a buffer one byte too small."""
+    import random
+    import mgx
+    L = mgx.lib()
+    cap = 8
+    rng = random.Random(4)
+    vals = [rng.getrandbits(rng.randint(1, 64 * cap)) * rng.choice([1, -1]) for _ in range(200)]
+    vals += [1 << 128, -(1 << 448), (1 << 512) - 1, 1 << 63, -(1 << 64)]
+    for v in vals:
+        e = to_entries([[v]], cap)[0, 0]
+        want = ("-" if v < 0 else "") + format(abs(v), "x")
+        buf = ctypes.create_string_buffer(200)
+        assert L.mgx_entry_to_hex(buf, 200, e.ctypes.data, cap) == len(want)
+        assert buf.value.decode() == want
+        assert L.mgx_entry_to_hex(buf, len(want), e.ctypes.data, cap) == -1   # no room for the NUL
+    e = to_entries([[-0x1234]], cap)[0, 0]
+    e[cap] = np.uint64((-3) & 0xFFFFFFFFFFFFFFFF)   # size -3, limbs 1 and 2 zero
+    buf = ctypes.create_string_buffer(64)
+    assert L.mgx_entry_to_hex(buf, 64, e.ctypes.data, cap) == 5 and buf.value == b"-1234"
+    z = np.zeros(cap + 1, dtype=np.uint64)
+    z[cap] = np.uint64((-2) & 0xFFFFFFFFFFFFFFFF)   # a negative zero prints as 0
+    assert L.mgx_entry_to_hex(buf, 64, z.ctypes.data, cap) == 1 and buf.value == b"0"
+
+
 def test_committed_cost_model_files_agree():
     """The text model (read by mgx_costmodel_load) and the JSON copy hold the
     same 3-layer, 64-hidden network."""
@@ -113,7 +140,7 @@ def test_committed_cost_model_files_agree():
         rows = [np.array(line.split(), dtype=np.float64) for line in f]
     assert head[:2] == ["mgx-costmodel", "1"]
     D, H, O = (int(x) for x in head[2:5])
-    assert (D, H, O) == (d["D"], d["H"], d["O"]) == (13, 64, 4)
+    assert (D, H, O) == (d["D"], d["H"], d["O"]) == (13, 64, 9)   # one output per schedule
     assert len(rows) == 3 and rows[0].size == D and rows[1].size == D
     assert rows[2].size == H * D + H + H * H + H + O * H + O
     assert np.allclose(rows[2], d["params"], rtol=1e-6) and np.allclose(rows[0], d["mean"], rtol=1e-6)

@@ -258,3 +258,33 @@ def test_committed_model_loads_and_dispatches(gpu):
         L.mgx_dispatch_set_model(None)
         L.mgx_costmodel_destroy(cm)
         L.mgx_costmodel_destroy(cm2)
+
+
+def test_device_dispatch_matches_host_inference(gpu):
+    """MGX_ROUTE_AUTO runs the features and the network on the device beside
+    its scan; its choice equals the host-side features + inference kernel
+    (mgx_dispatch_choose_schedule) on varied products, and the result is exact."""
+    import os
+    mgx = gpu
+    L = mgx.lib()
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "bachelor-thesis_amd")
+    cm = vp()
+    mgx.check(L.mgx_costmodel_load(ctypes.byref(cm), os.path.join(here, "costmodel_params.txt").encode()))
+    try:
+        mgx.check(L.mgx_dispatch_set_model(cm))
+        rng = np.random.default_rng(8)
+        for i in range(16):
+            K, M, N = (int(x) for x in rng.integers(2, 70, 3))
+            bits = int(2 ** rng.uniform(5, 13))
+            cap = (2 * bits + 8 + 63) // 64 + 2
+            A, B = rand_ints(K, M, bits, 600 + i), rand_ints(M, N, bits, 700 + i)
+            dA, dB = mgx.DeviceBuffer.from_array(to_entries(A, cap)), mgx.DeviceBuffer.from_array(to_entries(B, cap))
+            dC = mgx.DeviceBuffer(K * N * (cap + 1) * 8)
+            host = ctypes.c_int()
+            mgx.check(L.mgx_dispatch_choose_schedule(K, M, N, dA.ptr, M, dB.ptr, N, cap, ctypes.byref(host)))
+            mgx.check(L.mgx_matmul_route(-1, K, M, N, dC.ptr, N, dA.ptr, M, dB.ptr, N, cap, None))
+            assert L.mgx_dispatch_last() == host.value, (i, K, M, N, bits)
+            assert to_ints(dC.to_array((K, N, cap + 1))) == matmul(A, B)
+    finally:
+        L.mgx_dispatch_set_model(None)
+        L.mgx_costmodel_destroy(cm)
