@@ -117,7 +117,7 @@ class Problem:
 
 
 def sample_config(rng):
-    n = int(round(2 ** rng.uniform(4, 9)))          # 16 .. 512
+    n = int(round(2 ** rng.uniform(4, 10)))         # 16 .. 1024
     rect = rng.random() < 0.1
     K, M, N = n, (2 * n if rect else n), n
     blo = int(round(2 ** rng.uniform(6, 14)))       # 64 .. 16384 bits
@@ -285,7 +285,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--out", default=DEFAULT_PARAMS)
     ap.add_argument("--params", default=DEFAULT_PARAMS)
-    ap.add_argument("--data", default=None, help="save/load the timing data (npz)")
+    ap.add_argument("--data", default=None, help="save/load the timing data (npz; train: comma-separated list)")
     ap.add_argument("--json", default=None)
     a = ap.parse_args()
     mgx.init(0)
@@ -300,9 +300,12 @@ def main():
         print(json.dumps({"configs": int(X.shape[0]), "data": a.data}))
     elif a.cmd == "train":
         t0 = time.time()
-        if a.data and os.path.exists(a.data):
-            z = np.load(a.data)
-            X, T, flags = z["X"], z["T"], z["flags"]
+        paths = a.data.split(",") if a.data else []
+        if paths and all(os.path.exists(p) for p in paths):   # several timing files are concatenated
+            zs = [np.load(p) for p in paths]
+            X = np.concatenate([z["X"] for z in zs])
+            T = np.concatenate([z["T"] for z in zs])
+            flags = np.concatenate([z["flags"] for z in zs])
         else:
             X, T, flags, _ = generate(a.configs, a.seed)
             if a.data:

@@ -1034,15 +1034,37 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
         dst[u] = *(const v4i *)(Mc + (int64_t)(rb * RECON_ROWS + (ch >> 2)) * Kr + (k0_) + 16 * (ch & 3)); \
     }
     // B fragments straight from Y: K index 4w+b of an entry is byte b of word
-    // plane w.  Columns k >= S*P of Mc are zero, so planes past the last are
-    // read clamped to it (in bounds, multiplied by zero).
+    // plane w.  While the planes span less than 4 GB they are read through one
+    // buffer resource based at the workgroup's first entry (which the compiler
+    // can see is wave-uniform): a load is the lane's fixed offset (its wave's
+    // entries, entry column, half h: planes 4h .. 4h+3 of each group of 8)
+    // plus a wave-uniform plane offset, with no per-load address arithmetic,
+    // and planes past the last lie beyond the resource's range, where buffer
+    // loads return 0 (the Mc columns k >= S*P are zero anyway).  Larger Y
+    // regions are read with global loads, planes clamped to the last.
     const int nyp = (S * P + 3) / 4;
+    const uint32_t ysb = (uint32_t)ystride * 4;
+    const bool ybuf = (int64_t)nyp * ystride * 4 < (1ll << 32);
+    const uint32_t yvoff = (uint32_t)(64 * wave + c32) * 4 + 4u * (uint32_t)h * ysb;
+    // range: the bytes from entry e0 of plane 0 to the end of the last plane
+    // (below 2^32 whenever ybuf, and e0 < ystride), in 32-bit scalar arithmetic
+    const auto yrs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(Y + e0), 0, (int)((uint32_t)nyp * ysb - (uint32_t)e0 * 4u), BUF_DWORD3);
 #define YLOAD(dst, k0_)                                                                  \
-    _Pragma("unroll") for (int ct = 0; ct < 2; ct++)                                     \
-    _Pragma("unroll") for (int kk = 0; kk < 2; kk++)                                     \
-    _Pragma("unroll") for (int i = 0; i < 4; i++) {                                      \
-        const int w = min((k0_) / 4 + 8 * kk + 4 * h + i, nyp - 1);                      \
-        dst[ct][kk][i] = (int)yp[(int64_t)w * ystride + 32 * ct];                        \
+    if (ybuf) {                                                                          \
+        const uint32_t so_ = (uint32_t)((k0_) / 4) * ysb;                                \
+        _Pragma("unroll") for (int ct = 0; ct < 2; ct++)                                 \
+        _Pragma("unroll") for (int kk = 0; kk < 2; kk++)                                 \
+        _Pragma("unroll") for (int i = 0; i < 4; i++)                                    \
+            dst[ct][kk][i] = (int)__builtin_amdgcn_raw_buffer_load_b32(                  \
+                yrs, yvoff + 128 * ct, so_ + (8 * kk + i) * ysb, 0);                      \
+    } else {                                                                             \
+        _Pragma("unroll") for (int ct = 0; ct < 2; ct++)                                 \
+        _Pragma("unroll") for (int kk = 0; kk < 2; kk++)                                 \
+        _Pragma("unroll") for (int i = 0; i < 4; i++) {                                  \
+            const int w = min((k0_) / 4 + 8 * kk + 4 * h + i, nyp - 1);                  \
+            dst[ct][kk][i] = (int)yp[(int64_t)w * ystride + 32 * ct];                    \
+        }                                                                                \
     }
 #define RSTEP(ycur, ynext2)                                                                          \
     {                                                                                                \
@@ -1067,11 +1089,14 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
                     gsum[ct] += gamma_term<S>((uint32_t)ycur[ct][kk][i], sub, q, P, pc, f);          \
             }                                                                                        \
         }                                                                                            \
+        /* all eight A fragments of the step first, so that no MFMA waits on its own LDS read */     \
+        v4i af[2][4];                                                                                \
         _Pragma("unroll") for (int kk = 0; kk < 2; kk++)                                             \
-        _Pragma("unroll") for (int rt = 0; rt < 4; rt++) {                                           \
-            const v4i af = *(const v4i *)(LA + (rt * 32 + c32) * apitch + (RESIDENT ? k0 : 0) + kk * 32 + 16 * h); \
-            _Pragma("unroll") for (int ct = 0; ct < 2; ct++) acc[rt][ct] = mfma32(af, ycur[ct][kk], acc[rt][ct]); \
-        }                                                                                            \
+        _Pragma("unroll") for (int rt = 0; rt < 4; rt++)                                             \
+            af[kk][rt] = *(const v4i *)(LA + (rt * 32 + c32) * apitch + (RESIDENT ? k0 : 0) + kk * 32 + 16 * h); \
+        _Pragma("unroll") for (int kk = 0; kk < 2; kk++)                                             \
+        _Pragma("unroll") for (int rt = 0; rt < 4; rt++)                                             \
+        _Pragma("unroll") for (int ct = 0; ct < 2; ct++) acc[rt][ct] = mfma32(af[kk][rt], ycur[ct][kk], acc[rt][ct]); \
         k0 += 64;                                                                                    \
         if (k0 >= Kr) break;                                                                         \
     }
