@@ -32,6 +32,7 @@
 #include <cstring>
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <mutex>
 
 namespace mgx {
@@ -111,14 +112,6 @@ __global__ __launch_bounds__(256) void crt_reduce_kernel(
     reduce_tile<S>(blockIdx.x, pc, cst, P, nrb, kb, xbits, X, Xrs, W, transpose, rows, K, kpad, nent, R, plane, err);
 }
 
-// one operand of an entry-resident reduce launch
-struct ResSide {
-    const uint64_t *X;
-    int64_t Xrs, rows, plane;
-    int transpose;
-    int bits;   // |entries| < 2^bits (the plan's bound for this operand)
-    int8_t *R;
-};
 // Up to two operands (A and B of one product share the plan, K and kpad) in
 // one launch: workgroups [0, nblk0) reduce side 0, the rest side 1.
 template <int S, int KS>
@@ -130,6 +123,15 @@ __global__ __launch_bounds__(256) void crt_reduce_res_kernel(
     const ResSide &sd = second ? s1 : s0;
     reduce_tile_res<S, KS>(second ? blockIdx.x - nblk0 : blockIdx.x, pc, cst, P, nrb, kb, sd.bits, sd.X, sd.Xrs, W,
                            sd.transpose, sd.rows, K, kpad, sd.R, sd.plane, err);
+}
+// The constant-resident reduce: one workgroup per CU, units [0, nu0) of 256
+// entries (4 x 64 patches) from side 0, the rest from side 1.
+template <int KS>
+__global__ __launch_bounds__(CRES_THREADS, 1) void crt_reduce_cres_kernel(
+    const PrimeConst *__restrict__ pc, const int8_t *__restrict__ cst, int P, int kb, int64_t W, int64_t K,
+    int64_t kpad, ResSide s0, ResSide s1, int64_t nu0, int64_t nu, unsigned *__restrict__ err)
+{
+    reduce_cres<KS>(pc, cst, P, kb, W, K, kpad, s0, s1, nu0, nu, err);
 }
 // the entry-resident reduce for a plan's byte bound kb (<= RES_KB, a multiple of 64)
 template <int S>
@@ -435,6 +437,54 @@ int plan_create_slices(mgx_crt_plan **out, int64_t K, int64_t bitsA, int64_t bit
 }
 }  // namespace mgx
 
+namespace mgx {
+namespace {
+int device_cus()
+{
+    static std::atomic<int> cus[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    int n = cus[dev].load(std::memory_order_relaxed);
+    if (!n) {
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1) n = 256;
+        cus[dev].store(n, std::memory_order_relaxed);
+    }
+    return n;
+}
+}  // namespace
+
+// The reduce of one or two operands of a plan whose entries fit the
+// entry-resident tiles (kb <= RES_KB): nA (nB) padded entries of side a (b;
+// nB = 0: one side).  Two-digit plans whose constant table fits LDS take the
+// constant-resident kernel (one workgroup per CU), unless MGX_PARAM_REDUCE
+// asks for the tiles.
+int reduce_sides_launch(const mgx_crt_plan *pl, const ResSide &sa, const ResSide &sb, int64_t nA, int64_t nB,
+                        int64_t K, int64_t kpad, int64_t cap, hipStream_t s)
+{
+    const int64_t lds = cres_lds_bytes(pl->S, pl->P, pl->kb);
+    if (pl->S == 2 && pl->kb <= 256 && lds <= CRES_MAX_LDS && param(MGX_PARAM_REDUCE) == 0) {
+        auto kern = pl->kb <= 64 ? crt_reduce_cres_kernel<1> : pl->kb <= 128 ? crt_reduce_cres_kernel<2>
+                                                                              : crt_reduce_cres_kernel<4>;
+        MGX_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        // (nA and nB are multiples of 128 x 64: whole 4 x 64 patches)
+        const int64_t nu0 = nA / 256, nu = nu0 + nB / 256;
+        const int64_t grid = std::min<int64_t>(device_cus(), nu);
+        if (grid > 0)
+            hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(CRES_THREADS), (unsigned)lds, s, pl->d_pc, pl->d_cst,
+                               (int)pl->P, (int)pl->kb, cap + 1, K, kpad, sa, sb, nu0, nu, device_error_word());
+        MGX_HIP(hipGetLastError());
+        return MGX_OK;
+    }
+    auto res = pl->S == 2 ? reduce_res_for<2>(pl->kb) : reduce_res_for<4>(pl->kb);
+    const int64_t nblk0 = ceil_div(nA, 128), nblk = nblk0 + ceil_div(nB, 128);
+    if (nblk > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "matrices too large for one reduce launch");
+    hipLaunchKernelGGL(res, dim3((unsigned)nblk), dim3(256), 0, s, pl->d_pc, pl->d_cst, (int)pl->P,
+                       (int)(pl->rows4pad / 64), (int)pl->kb, cap + 1, K, kpad, sa, sb, nblk0, device_error_word());
+    MGX_HIP(hipGetLastError());
+    return MGX_OK;
+}
+}  // namespace mgx
+
 extern "C" {
 
 int mgx_crt_plan_destroy(mgx_crt_plan *pl)
@@ -484,8 +534,7 @@ int mgx_crt_reduce(const mgx_crt_plan *pl, int transpose, int64_t rows, int64_t 
     auto gen = pl->S == 2 ? crt_reduce_kernel<2> : crt_reduce_kernel<4>;
     if (resident) {
         const ResSide sd{(const uint64_t *)X, Xrs, rows, plane, transpose, xbits, R};
-        hipLaunchKernelGGL(res, dim3((unsigned)nblk), dim3(256), 0, as_stream(stream), pl->d_pc, pl->d_cst,
-                           (int)pl->P, nrb, (int)pl->kb, cap + 1, K, kpad, sd, sd, nblk, device_error_word());
+        return reduce_sides_launch(pl, sd, sd, nent, 0, K, kpad, cap, as_stream(stream));
     } else
         hipLaunchKernelGGL(gen, dim3((unsigned)nblk), dim3(256), 0, as_stream(stream), pl->d_pc, pl->d_cst,
                            (int)pl->P, nrb, (int)pl->kb, xbits, (const uint64_t *)X, Xrs, cap + 1, transpose, rows, K, kpad,
@@ -717,13 +766,7 @@ int crt_launch(const mgx_crt_plan *pl, int64_t K, int64_t M, int64_t N, mgx_limb
         const int64_t nA = L.rpadA * L.kpad, nB = L.rpadB * L.kpad;
         const ResSide sa{(const uint64_t *)A, Ars, K, nA, 0, (int)pl->bitsA, RA},
             sb{(const uint64_t *)B, Brs, N, nB, 1, (int)pl->bitsB, RB};
-        auto res = pl->S == 2 ? reduce_res_for<2>(pl->kb) : reduce_res_for<4>(pl->kb);
-        const int64_t nblk = ceil_div(nA, 128) + ceil_div(nB, 128);
-        if (nblk > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "matrices too large for one reduce launch");
-        hipLaunchKernelGGL(res, dim3((unsigned)nblk), dim3(256), 0, s, pl->d_pc, pl->d_cst, (int)pl->P,
-                           (int)(pl->rows4pad / 64), (int)pl->kb, cap + 1, M, L.kpad, sa, sb, ceil_div(nA, 128),
-                           device_error_word());
-        MGX_HIP(hipGetLastError());
+        MGX_TRY(reduce_sides_launch(pl, sa, sb, nA, nB, M, L.kpad, cap, s));
     } else {
         MGX_TRY(mgx_crt_reduce(pl, 0, K, M, A, Ars, cap, 0, RA, stream));
         MGX_TRY(mgx_crt_reduce(pl, 1, N, M, B, Brs, cap, 0, RB, stream));

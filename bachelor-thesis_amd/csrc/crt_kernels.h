@@ -467,6 +467,200 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
 }
 
 // ------------------------------------------------------------------------
+// 1b. constant-resident reduce (S = 2, entries of at most 256 bytes).  One
+//     workgroup of 8 waves per CU keeps the whole constant table (2P rows x
+//     kb bytes) and the per-prime epilogue constants in LDS, loaded once.
+//     The workgroup takes a 4-row x 64-column patch of a residue plane (256
+//     entries) at a time; wave w holds 32 of them (patch row w/2, columns
+//     32 (w%2) .. + 31), the lane's half of an entry's bytes in registers as
+//     the MFMA A operand (rows = entries).  The constants are the B operand:
+//     row block rb of the table holds digit 0 of primes 32 rb .. 32 rb + 31,
+//     then digit 1 of the same primes, so that a lane's output column is one
+//     prime (both digits in the same register of the two 32-column tiles)
+//     and its epilogue constants are read once per row block.  The residues
+//     of a row block go through a plane-major LDS tile (64 planes x 256
+//     entries) and leave as 256 contiguous bytes per plane (a 4 x 64 patch
+//     of the 16 x 64 blocked plane), two barriers per row block.
+//     Table layout (rows of 64 KS bytes, zero past kb): row block rb < nrb - 1
+//     at rows 64 rb + 32 d + q % 32; the
+//     last block's nlast primes at 64 (nrb - 1) + nlast d + q % 32 (2P rows in
+//     all); 16-byte chunk c of row r at slot c ^ cres_swz(r), so that the rows
+//     one fragment read touches per LDS cycle hit distinct banks for kb of 64,
+//     128 or 256 bytes.
+// ------------------------------------------------------------------------
+// one operand of a reduce launch that takes A and B together
+struct ResSide {
+    const uint64_t *X;
+    int64_t Xrs, rows, plane;
+    int transpose;
+    int bits;   // |entries| < 2^bits (the plan's bound for this operand)
+    int8_t *R;
+};
+
+constexpr int CRES_THREADS = 512;              // eight waves, two per SIMD
+constexpr int CRES_LO_PITCH = 260;             // bytes per plane of the output tile (bank shift 1)
+constexpr int CRES_MAX_LDS = 160 * 1024;
+// LDS row of the table: kb rounded up to 64, 128 or 256 bytes (the columns
+// past kb are zero: the entries' bytes there enter as -128)
+inline int64_t cres_row_bytes(int64_t kb) { return kb <= 64 ? 64 : kb <= 128 ? 128 : 256; }
+inline int64_t cres_lds_bytes(int64_t S, int64_t P, int64_t kb)
+{
+    return S * P * cres_row_bytes(kb) + round_up(P, 32) * 16 + 64 * CRES_LO_PITCH;
+}
+__device__ __forceinline__ int cres_swz(int r, int C) { return ((r * C) >> 4) & (C - 1); }
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+template <int KS>
+__device__ __forceinline__ void reduce_cres(const PrimeConst *__restrict__ pc, const int8_t *__restrict__ cst, int P,
+    int kb, int64_t W, int64_t K, int64_t kpad, const ResSide &s0, const ResSide &s1, int64_t nu0, int64_t nu,
+    unsigned *__restrict__ err)
+{
+    static_assert(KS >= 1 && KS <= 4, "entries of at most 256 bytes");
+    constexpr int K32 = 2 * KS, C = 4 * KS;   // 32-byte MFMA steps; 16-byte chunks per row
+    extern __shared__ __attribute__((aligned(16))) int8_t LDYN[];
+    const int SP = 2 * P, nrb = (P + 31) / 32, lastq = 32 * (nrb - 1), lastrow = 2 * lastq, nlast = P - lastq;
+    int8_t *const LC = LDYN;
+    uint4 *const s_pc = (uint4 *)(LDYN + (int64_t)SP * 64 * KS);
+    uint8_t *const LO = (uint8_t *)(s_pc + 32 * nrb);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c32 = lane & 31, hh = lane >> 5;
+    {
+        const int nch = SP * C;
+        for (int c0 = 0; c0 < nch; c0 += 8 * CRES_THREADS) {
+            v4i t[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int c = min(c0 + tid + CRES_THREADS * u, nch - 1), lr = c / C, ch = c - lr * C;
+                int q, d;
+                if (lr < lastrow) { q = (lr >> 6) * 32 + (lr & 31); d = (lr >> 5) & 1; }
+                else { const int t_ = lr - lastrow; d = t_ / nlast; q = lastq + t_ - d * nlast; }
+                t[u] = 16 * ch < kb ? *(const v4i *)(cst + (int64_t)(2 * q + d) * kb + 16 * ch) : (v4i){0, 0, 0, 0};
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int c = c0 + tid + CRES_THREADS * u, lr = c / C, ch = c - lr * C;
+                if (c < nch) *(v4i *)(LC + lr * 64 * KS + 16 * (ch ^ cres_swz(lr, C))) = t[u];
+            }
+        }
+        // per prime: -p, floor(2^32/p), and the offsets K + off32 + h of a
+        // positive and of a negative entry (h = floor(p/2)); rows past the last
+        // prime repeat it
+        for (int i = tid; i < 32 * nrb; i += CRES_THREADS) {
+            const PrimeConst &c_ = pc[min(i, P - 1)];
+            const uint32_t h_ = c_.p >> 1;
+            s_pc[i] = make_uint4((uint32_t)-(int32_t)c_.p, (uint32_t)(c_.mu >> 32), c_.kofs + c_.off32 + h_,
+                                 c_.kneg + c_.off32 + h_);
+        }
+        __syncthreads();
+    }
+    // unit uu: a 4-row x 64-column patch of side 0 (uu < nu0) or side 1;
+    // this lane's entry: limb base (padding reads entry 0), size word, limbs
+    uint64_t raw[K32][2];
+    int64_t rsz = 0;
+    bool rreal = false;
+    auto load = [&](int64_t uu) {
+        const bool s = uu >= nu0;
+        const int64_t ul = uu - (s ? nu0 : 0), cbs = kpad >> 6;
+        const int64_t r = 4 * (ul / cbs) + (wave >> 1), k = 64 * (ul % cbs) + 32 * (wave & 1) + c32;
+        const int64_t Xrs = s ? s1.Xrs : s0.Xrs;
+        rreal = r < (s ? s1.rows : s0.rows) && k < K;
+        const int64_t base = rreal ? ((s ? s1.transpose : s0.transpose) ? (k * Xrs + r) * W : (r * Xrs + k) * W) : 0;
+        const uint64_t *X = s ? s1.X : s0.X;
+        const int lim = (int)min<int64_t>(kb / 8, W - 1);
+        rsz = (int64_t)X[base + W - 1];
+#pragma unroll
+        for (int k32 = 0; k32 < K32; k32++)
+#pragma unroll
+            for (int u = 0; u < 2; u++) raw[k32][u] = X[base + min(4 * k32 + 2 * hh + u, lim - 1)];
+        if (hh == 0 && rreal && over_bound(X + base, min(rsz < 0 ? -rsz : rsz, W - 1), s ? s1.bits : s0.bits))
+            atomicOr(err, DEV_ERR_INPUT);
+    };
+    int64_t uu = blockIdx.x;
+    if (uu < nu) load(uu);
+    for (; uu < nu; uu += gridDim.x) {
+        // bytes - 128, or 127 - byte for a negative entry (the products then sum
+        // to -|x| plus a constant), masked at |size|
+        const bool neg = rreal && rsz < 0;
+        v4i af[K32];
+        {
+            const int nl = rreal ? (int)min<int64_t>(min(rsz < 0 ? -rsz : rsz, W - 1), (int64_t)(kb / 8)) : 0;
+            const uint64_t flip = neg ? 0x7f7f7f7f7f7f7f7full : 0x8080808080808080ull;
+#pragma unroll
+            for (int k32 = 0; k32 < K32; k32++) {
+                const uint64_t d0 = (4 * k32 + 2 * hh < nl ? raw[k32][0] : 0) ^ flip;
+                const uint64_t d1 = (4 * k32 + 2 * hh + 1 < nl ? raw[k32][1] : 0) ^ flip;
+                af[k32] = (v4i){(int)(uint32_t)d0, (int)(uint32_t)(d0 >> 32), (int)(uint32_t)d1, (int)(uint32_t)(d1 >> 32)};
+            }
+        }
+        // output register 4g + i of this lane belongs to entry 8g + 4hh + i of
+        // the wave; bit 4g + i of m16: that entry is negative
+        uint32_t m16;
+        {
+            const uint32_t nm = (uint32_t)__ballot(neg) >> (4 * hh);   // bit e: entry e (lanes e and e + 32 agree)
+            m16 = (nm & 0xfu) | ((nm >> 4) & 0xf0u) | ((nm >> 8) & 0xf00u) | ((nm >> 12) & 0xf000u);
+        }
+        const bool s = uu >= nu0;
+        const int64_t ul = uu - (s ? nu0 : 0), cbs = kpad >> 6;
+        int8_t *const R = s ? s1.R : s0.R;
+        const int64_t plane = s ? s1.plane : s0.plane;
+        if (uu + gridDim.x < nu) load(uu + gridDim.x);
+        // store phase: thread t takes plane t / 8 of the row block and 32
+        // entries of the patch: row (t % 8) / 2, columns 32 (t % 2) .. + 31
+        const int sp = tid >> 3, sj = tid & 7;
+        int8_t *const Rb = R + res_off(2, 4 * (ul / cbs) + (sj >> 1), 64 * (ul % cbs) + 32 * (sj & 1), kpad);
+        for (int rb = 0; rb < nrb; rb++) {
+            v16i acc[2];   // acc[d]: the digit-d column of this lane's prime
+#pragma unroll
+            for (int k32 = 0; k32 < K32; k32++)
+#pragma unroll
+                for (int d = 0; d < 2; d++) {
+                    const int r = rb < nrb - 1 ? rb * 64 + d * 32 + c32 : lastrow + d * nlast + min(c32, nlast - 1);
+                    const v4i bfr = *(const v4i *)(LC + r * 64 * KS + 16 * ((2 * k32 + hh) ^ cres_swz(r, C)));
+                    acc[d] = mfma32(af[k32], bfr, k32 == 0 ? (v16i){0} : acc[d]);
+                }
+            // |acc| <= kb 2^14 <= 2^22, so v = acc_0 + 256 acc_1 is below 2^31 -
+            // 2^22 in magnitude and congruent to x - 128 sum 2^(8b) (to -x + 127
+            // sum 2^(8b) for a negative entry); u = v + K with K = (128 resp.
+            // -127) sum 2^(8b) mod p + off32 + h lies in [0, 2^32) and is
+            // congruent to x + h, so (u mod p) - h is the centred residue and
+            // (u mod p) + 128 - h carries its two balanced digits: byte 0 ^ 0x80
+            // and byte 1.  Padding entries have no limbs and come out 0.
+            const uint4 k4 = s_pc[rb * 32 + c32];   // -p, mu32, K (positive), K (negative)
+            const uint32_t c128h = 128u - ((uint32_t)-(int32_t)k4.x >> 1), dk = k4.w - k4.z;
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                uint32_t v[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const uint32_t sel = (uint32_t)__builtin_amdgcn_sbfe((int)m16, 4 * g + i, 1);   // 0 or ~0
+                    const uint32_t u = (uint32_t)(acc[0][4 * g + i] + (acc[1][4 * g + i] << 8)) + k4.z + (dk & sel);
+                    v[i] = bred32(u, (int32_t)k4.x, k4.y) + c128h;
+                }
+                // digit d of entries 8g + 4hh .. + 3: byte d of v[0..3]
+                const uint32_t t01 = __builtin_amdgcn_perm(v[1], v[0], 0x05010400u);
+                const uint32_t t23 = __builtin_amdgcn_perm(v[3], v[2], 0x05010400u);
+                uint8_t *const lo = LO + 2 * c32 * CRES_LO_PITCH + 32 * wave + 8 * g + 4 * hh;
+                *(uint32_t *)lo = __builtin_amdgcn_perm(t23, t01, 0x05040100u) ^ 0x80808080u;
+                *(uint32_t *)(lo + CRES_LO_PITCH) = __builtin_amdgcn_perm(t23, t01, 0x07060302u);
+            }
+            lds_barrier();
+            {
+                const uint32_t *src = (const uint32_t *)(LO + sp * CRES_LO_PITCH + 32 * sj);
+                uint32_t w8[8];
+#pragma unroll
+                for (int x = 0; x < 8; x++) w8[x] = src[x];
+                const int grow = rb * 64 + sp;   // plane 2 q + d
+                if (grow < SP) {
+                    *(uint4 *)(Rb + grow * plane) = make_uint4(w8[0], w8[1], w8[2], w8[3]);
+                    *(uint4 *)(Rb + grow * plane + 16) = make_uint4(w8[4], w8[5], w8[6], w8[7]);
+                }
+            }
+            lds_barrier();
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
 // 2. residue GEMM.  Tile 128 (i) x 64 (j) per prime; 4 waves of 64 x 32.
 // ------------------------------------------------------------------------
 constexpr int KCHUNK = 8192;   // 4 pairs * 8192 * 2^14 = 2^29 < 2^31

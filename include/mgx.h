@@ -280,8 +280,11 @@ typedef enum {
     MGX_PARAM_TRANSFER = 4,       /* mgx_matmul_host: largest block side moved to the device at once
                                      (reference TRANSFER_SIZE = 512, mat-gpu/mat-params.h:6); 0 = the whole
                                      product (default) */
-    MGX_PARAM_GEMM_ROWS = 5       /* residue-GEMM tile height of two-digit plans: 0 or 128 (default), or 64
+    MGX_PARAM_GEMM_ROWS = 5,      /* residue-GEMM tile height of two-digit plans: 0 or 128 (default), or 64
                                      (twice the workgroups, for small products) */
+    MGX_PARAM_REDUCE = 6          /* reduce kernel of two-digit plans with entries of at most 256 bytes:
+                                     0 = one workgroup per CU with the whole constant table in LDS
+                                     whenever it fits (default), 1 = entry-resident tiles of 128 entries */
 } mgx_param;
 int mgx_set_param(int param, int64_t value);
 int64_t mgx_get_param(int param);
