@@ -95,6 +95,18 @@ __global__ __launch_bounds__(256) void batch_reduce_res_kernel(Queue q, const Re
 // reduce tile kind for a byte bound: 0 entry-resident up to 256 bytes, 1 generic, 2 entry-resident up to 512
 inline int reduce_kind(int64_t kb) { return kb <= 256 ? 0 : kb <= RES_KB ? 2 : 1; }
 
+// two-digit plans with entries over 512 bytes: the reduce as a tiled MFMA GEMM
+__global__ __launch_bounds__(256, 2) void batch_reduce_gemm_kernel(Queue q, const ReduceJob *jobs, unsigned *err)
+{
+    int j;
+    int64_t t;
+    while (next_item(q, j, t)) {
+        const ReduceJob &b = jobs[j];
+        reduce_gemm_tile(t, b.pc, b.cst, b.P, b.kb, b.xbits, b.X, b.Xrs, b.W, b.transpose, b.rows, b.K, b.kpad, b.nent,
+                         b.R, b.plane, err);
+    }
+}
+
 template <int S>
 __global__ __launch_bounds__(256) void batch_reduce_gen_kernel(Queue q, const ReduceJob *jobs, unsigned *err)
 {
@@ -272,6 +284,8 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
     // chunking (2 sc + chunked); the carry pass takes every CRT product in one
     // queue.
     std::vector<ReduceJob> rj[6];
+    // two-digit generic reduce jobs (class 3) as tiled MFMA GEMMs (fixed at creation: the graph is captured here)
+    const bool rgemm = param(MGX_PARAM_REDUCE) == 0;
     std::vector<GemmJob> gj[4];
     std::vector<ReconJob> cj[2], kj;
     std::vector<PlainJob> pj[NCLASS];
@@ -352,7 +366,8 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
                 r.R = (int8_t *)(base + (t ? o_rb[i] : o_ra[i]));
                 const int rc = 2 * kind + sc;
                 rj[rc].push_back(r);
-                rp[rc].push_back(rp[rc].back() + ceil_div(r.nent, 128) * (resident ? 1 : nrb));
+                rp[rc].push_back(rp[rc].back() + (rc == 3 && rgemm ? rg_tiles(r.nent, r.P)
+                                                                    : ceil_div(r.nent, 128) * (resident ? 1 : nrb)));
             }
             GemmJob g;
             g.pc = v.pc; g.RA = (const int8_t *)(base + o_ra[i]); g.RB = (const int8_t *)(base + o_rb[i]);
@@ -446,7 +461,9 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             if (rj[c].empty()) continue;
             const int kind = c >> 1, two = c & 1;
             int grid = (int)std::min<int64_t>(rp[c].back(), kind == 1 ? 2048 : 1024);
-            auto kern = kind == 1 ? (two ? batch_reduce_gen_kernel<2> : batch_reduce_gen_kernel<4>)
+            if (c == 3 && rgemm) grid = (int)std::min<int64_t>(rp[c].back(), 512);
+            auto kern = c == 3 && rgemm ? batch_reduce_gemm_kernel
+                      : kind == 1 ? (two ? batch_reduce_gen_kernel<2> : batch_reduce_gen_kernel<4>)
                       : kind == 2 ? (two ? batch_reduce_res_kernel<2, 8> : batch_reduce_res_kernel<4, 8>)
                                   : (two ? batch_reduce_res_kernel<2, 4> : batch_reduce_res_kernel<4, 4>);
             const int slot = kind == 0 ? two : kind == 1 ? 15 + two : 17 + two;

@@ -124,6 +124,19 @@ __global__ __launch_bounds__(256) void crt_reduce_res_kernel(
     reduce_tile_res<S, KS>(second ? blockIdx.x - nblk0 : blockIdx.x, pc, cst, P, nrb, kb, sd.bits, sd.X, sd.Xrs, W,
                            sd.transpose, sd.rows, K, kpad, sd.R, sd.plane, err);
 }
+// The reduce of wide entries as a tiled MFMA GEMM (two-digit plans), tiles
+// in XCD-aware order: the tiles an XCD runs together share entry and table
+// tiles in its L2.
+__global__ __launch_bounds__(256, 2) void crt_reduce_gemm_kernel(
+    const PrimeConst *__restrict__ pc, const int8_t *__restrict__ cst, int P, int kb, int xbits,
+    const uint64_t *__restrict__ X, int64_t Xrs, int64_t W, int transpose, int64_t rows, int64_t K, int64_t kpad,
+    int64_t nent, int8_t *__restrict__ R, int64_t plane, int64_t total, unsigned *__restrict__ err)
+{
+    const int64_t per = (total + 7) / 8;
+    const int64_t v = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (v >= total) return;
+    reduce_gemm_tile(v, pc, cst, P, kb, xbits, X, Xrs, W, transpose, rows, K, kpad, nent, R, plane, err);
+}
 // The constant-resident reduce: one workgroup per CU, units [0, nu0) of 256
 // entries (4 x 64 patches) from side 0, the rest from side 1.
 template <int KS>
@@ -535,6 +548,12 @@ int mgx_crt_reduce(const mgx_crt_plan *pl, int transpose, int64_t rows, int64_t 
     if (resident) {
         const ResSide sd{(const uint64_t *)X, Xrs, rows, plane, transpose, xbits, R};
         return reduce_sides_launch(pl, sd, sd, nent, 0, K, kpad, cap, as_stream(stream));
+    } else if (pl->S == 2 && param(MGX_PARAM_REDUCE) == 0) {   // wide entries: the reduce as a tiled MFMA GEMM
+        const int64_t total = rg_tiles(nent, pl->P), grid = round_up(total, 8);
+        if (grid > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "matrix too large for one reduce launch");
+        hipLaunchKernelGGL(crt_reduce_gemm_kernel, dim3((unsigned)grid), dim3(256), 0, as_stream(stream), pl->d_pc,
+                           pl->d_cst, (int)pl->P, (int)pl->kb, xbits, (const uint64_t *)X, Xrs, cap + 1, transpose, rows,
+                           K, kpad, nent, R, plane, total, device_error_word());
     } else
         hipLaunchKernelGGL(gen, dim3((unsigned)nblk), dim3(256), 0, as_stream(stream), pl->d_pc, pl->d_cst,
                            (int)pl->P, nrb, (int)pl->kb, xbits, (const uint64_t *)X, Xrs, cap + 1, transpose, rows, K, kpad,

@@ -1039,6 +1039,219 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
 }
 
 // ------------------------------------------------------------------------
+// 1c. reduce as an MFMA GEMM (S = 2, entries wider than the entry-resident
+//     tiles take).  The residues of an entry are sum_b byte_b (digits of
+//     2^(8b) mod p), a product [entries x kb] x [kb x 2P]; for wide entries
+//     this is the largest MFMA work of a product (2 S P kb ops per entry), so
+//     it runs as a tiled GEMM fed by LDS DMA like the residue GEMM.
+//     Workgroup tile: 128 entries (a 2-row x 64-column patch of the residue
+//     plane) x 256 table rows (128 primes in groups of 32: the digit-0 rows,
+//     then the digit-1 rows of the same primes); 4 waves of 64 entries x 128
+//     table rows.  The entries are the MFMA A operand, so a lane's output
+//     column is one prime's digit and both digits of a prime meet in one
+//     register of two column tiles.  K runs in 64-byte steps through a
+//     RG_STAGES-deep LDS ring; the entries' 16-byte chunks come straight from
+//     their limbs (per-lane 64-bit addresses, kept inside the entry) and are
+//     masked at |size| and biased (byte - 128, or 127 - byte for a negative
+//     entry) when a wave reads its fragments.  Stage layout: [entries 128 x
+//     64 B][table 256 x 64 B], chunk c of row r at slot c ^ g2_swz(r).  The
+//     epilogue is the constant-resident reduce's with the digit-1 sums
+//     reduced first (|acc| <= kb 2^14 < 2^27 for the widths two-digit plans
+//     allow); the residues leave through LDS as 128 contiguous bytes per
+//     plane.  Tile order: groups of RG_GE entry tiles; inside a group the
+//     entry tile varies fastest, so the workgroups an XCD runs at once share
+//     a few entry tiles and a few table tiles in its L2.
+// ------------------------------------------------------------------------
+constexpr int RG_STAGES = 3;
+constexpr int RG_STAGE = 128 * 64 + 256 * 64;   // 24 KB
+constexpr int RG_OPITCH = 144;                  // bytes per plane of the output tile
+constexpr int RG_GE = 8;                        // entry tiles per group
+static_assert(RG_STAGES * RG_STAGE >= 256 * RG_OPITCH, "the output tile reuses the ring");
+inline int64_t rg_tiles(int64_t nent, int64_t P) { return ceil_div(nent, 128) * ceil_div(P, 128); }
+
+// 16 bytes from the lane's own address to LDS at lds_base + 16 * lane
+__device__ __forceinline__ void dma16v(const void *addr, uint32_t lds_base)
+{
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(lds_base), "v"(addr) : "memory", "m0");
+}
+
+__device__ __forceinline__ void reduce_gemm_tile(int64_t v, const PrimeConst *__restrict__ pc,
+    const int8_t *__restrict__ cst, int P, int kb, int xbits, const uint64_t *__restrict__ X, int64_t Xrs, int64_t W,
+    int transpose, int64_t rows, int64_t K, int64_t kpad, int64_t nent, int8_t *__restrict__ R, int64_t plane,
+    unsigned *__restrict__ err)
+{
+    constexpr int PER = 6;   // DMA instructions per wave and stage: 2 of entries, 4 of the table
+    __shared__ __attribute__((aligned(1024))) int8_t LS[RG_STAGES * RG_STAGE];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wave_u = __builtin_amdgcn_readfirstlane(wave);
+    const int wi = wave >> 1, wj = wave & 1, c32 = lane & 31, h = lane >> 5;
+    // tile v -> (entry tile et, table tile ct)
+    const int64_t nct = (P + 127) / 128, net = (nent + 127) / 128, gsz = RG_GE * nct;
+    const int64_t grp = v / gsz, inner = v % gsz, ge = min<int64_t>(RG_GE, net - grp * RG_GE);
+    const int64_t ct = inner / ge, et = grp * RG_GE + inner % ge, cbs = kpad >> 6;
+    const int64_t r0 = 2 * (et / cbs), col0 = 64 * (et % cbs);   // the patch: rows r0, r0 + 1, columns col0 ..
+    const int nsteps = kb / 64;
+    const int lim = (int)min<int64_t>(kb / 8, W - 1);   // limbs an entry can contribute
+    const uint32_t ls0 = __builtin_amdgcn_readfirstlane(lds_offset(LS));
+
+    // entry el = 64 row + column of the patch: limb base (padding: entry 0),
+    // limbs below |size| that count, sign
+    auto entry = [&](int el, int64_t &base, int &nl, bool &neg) {
+        const int64_t r = r0 + (el >> 6), k = col0 + (el & 63);
+        const bool real = r < rows && k < K;
+        base = real ? (transpose ? (k * Xrs + r) * W : (r * Xrs + k) * W) : 0;
+        const int64_t sz = (int64_t)X[base + W - 1];
+        const int64_t a = real ? min(sz < 0 ? -sz : sz, W - 1) : 0;
+        nl = (int)min<int64_t>(a, lim);
+        neg = real && sz < 0;
+    };
+    // DMA sources: this wave's entry instructions fill entry rows 32 wave + 16 u
+    // .. + 15, its table instructions table rows 64 wave + 16 u ..; lane l takes
+    // row + l / 4, slot l % 4, i.e. chunk (l % 4) ^ g2_swz(row) = (l % 4) ^ ((l >> 4) & 3)
+    const int chunk = (lane & 3) ^ ((lane >> 4) & 3);
+    const int8_t *eaddr[2];
+    uint32_t toff[4];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        int64_t b;
+        int nl_;
+        bool ng;
+        entry(32 * wave + 16 * u + (lane >> 2), b, nl_, ng);
+        eaddr[u] = (const int8_t *)(X + b) + 16 * chunk;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int row = 64 * wave + 16 * u + (lane >> 2);
+        const int64_t q = 128 * ct + 32 * (row >> 6) + (row & 31), d = (row >> 5) & 1;
+        toff[u] = (uint32_t)((q < P ? 2 * q + d : 0) * kb + 16 * chunk);
+    }
+    // bytes of an entry at offsets >= emax come from its last 16 bytes (masked)
+    const int64_t emax = 8 * W - 16;
+#define ISSUE(k_)                                                                                       \
+    {                                                                                                   \
+        const uint32_t sb_ = ls0 + ((k_) % RG_STAGES) * RG_STAGE;                                       \
+        const int64_t ko_ = (int64_t)(k_) * 64;                                                         \
+        _Pragma("unroll") for (int u = 0; u < 2; u++)                                                   \
+            dma16v(eaddr[u] + min<int64_t>(ko_, emax - 16 * chunk),                                     \
+                   __builtin_amdgcn_readfirstlane(sb_ + (32 * wave_u + 16 * u) * 64));                  \
+        const int8_t *tb_ = uniform_ptr(cst + ko_);                                                     \
+        _Pragma("unroll") for (int u = 0; u < 4; u++)                                                   \
+            dma16(tb_, toff[u], __builtin_amdgcn_readfirstlane(sb_ + 128 * 64 + (64 * wave_u + 16 * u) * 64)); \
+    }
+
+    // the fragments' entries (rows 64 wi + 32 rt + c32): limbs below |size| and sign
+    int fnl[2];
+    bool fneg[2];
+#pragma unroll
+    for (int rt = 0; rt < 2; rt++) {
+        int64_t b;
+        entry(64 * wi + 32 * rt + c32, b, fnl[rt], fneg[rt]);
+        if (ct == 0 && h == 0) {
+            const int64_t r = r0 + ((64 * wi + 32 * rt + c32) >> 6), k = col0 + ((64 * wi + 32 * rt + c32) & 63);
+            if (r < rows && k < K) {
+                const int64_t sz = (int64_t)X[b + W - 1];
+                if (over_bound(X + b, min(sz < 0 ? -sz : sz, W - 1), xbits)) atomicOr(err, DEV_ERR_INPUT);
+            }
+        }
+    }
+
+    v16i acc[2][4];
+#pragma unroll
+    for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[rt][c] = (v16i){0};
+#pragma unroll
+    for (int k = 0; k < RG_STAGES - 1; k++)
+        if (k < nsteps) { ISSUE(k); }
+    for (int k = 0; k < nsteps; k++) {
+        wait_stages<PER>(min(nsteps - 1, k + RG_STAGES - 2) - k);
+        __syncthreads();
+        if (k + RG_STAGES - 1 < nsteps) { ISSUE(k + RG_STAGES - 1); }
+        const int8_t *se = LS + (k % RG_STAGES) * RG_STAGE, *st = se + 128 * 64;
+#pragma unroll
+        for (int kk = 0; kk < 2; kk++) {
+            const int ch = 2 * kk + h;
+            v4i af[2], bfr[4];
+#pragma unroll
+            for (int rt = 0; rt < 2; rt++) {
+                const int row = 64 * wi + 32 * rt + c32;
+                const v4i raw = *(const v4i *)(se + row * 64 + 16 * (ch ^ g2_swz(row)));
+                // limbs 8k + 4kk + 2h and + 1 of the entry; masked at |size|, then biased
+                const int l0 = 8 * k + 4 * kk + 2 * h;
+                const uint32_t flip = fneg[rt] ? 0x7f7f7f7fu : 0x80808080u;
+                af[rt] = (v4i){(int)((l0 < fnl[rt] ? (uint32_t)raw[0] : 0u) ^ flip),
+                               (int)((l0 < fnl[rt] ? (uint32_t)raw[1] : 0u) ^ flip),
+                               (int)((l0 + 1 < fnl[rt] ? (uint32_t)raw[2] : 0u) ^ flip),
+                               (int)((l0 + 1 < fnl[rt] ? (uint32_t)raw[3] : 0u) ^ flip)};
+            }
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const int row = 128 * wj + 32 * c + c32;
+                bfr[c] = *(const v4i *)(st + row * 64 + 16 * (ch ^ g2_swz(row)));
+            }
+#pragma unroll
+            for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) acc[rt][c] = mfma32(af[rt], bfr[c], acc[rt][c]);
+        }
+    }
+#undef ISSUE
+    __syncthreads();   // the output tile reuses the ring
+
+    // epilogue: lane column c32 of column tiles 2 pg and 2 pg + 1 is prime
+    // q = 128 ct + 64 wj + 32 pg + c32 (digits 0 and 1); register 4g + i of
+    // row tile rt is entry 64 wi + 32 rt + 8g + 4h + i of the patch
+    uint8_t *const LO = (uint8_t *)LS;
+#pragma unroll
+    for (int pg = 0; pg < 2; pg++) {
+        const int64_t q = min<int64_t>(128 * ct + 64 * wj + 32 * pg + c32, P - 1);
+        const PrimeConst &c_ = pc[q];
+        const uint32_t hp = c_.p >> 1, mu32 = (uint32_t)(c_.mu >> 32), c128h = 128u - hp;
+        const int32_t negp = -(int32_t)c_.p;
+        const uint32_t kpos = c_.kofs + c_.off32 + hp, dk = c_.kneg + c_.off32 + hp - kpos;
+#pragma unroll
+        for (int rt = 0; rt < 2; rt++) {
+            uint32_t m16;
+            {
+                const uint32_t nm = (uint32_t)__ballot(fneg[rt]) >> (4 * h);   // bit e: entry 32 rt + e (of the wave's 64)
+                m16 = (nm & 0xfu) | ((nm >> 4) & 0xf0u) | ((nm >> 8) & 0xf00u) | ((nm >> 12) & 0xf000u);
+            }
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                uint32_t vv[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const uint32_t sel = (uint32_t)__builtin_amdgcn_sbfe((int)m16, 4 * g + i, 1);
+                    // |acc| < 2^27: the digit-1 sum reduced below 2p first, then
+                    // u = acc_0 + 256 r1 + K lies in [0, 2^32)
+                    const uint32_t r1 = smod32_lazy(acc[rt][2 * pg + 1][4 * g + i], negp, mu32, c_.off32);
+                    const uint32_t u = (uint32_t)(acc[rt][2 * pg][4 * g + i] + (int32_t)(r1 << 8)) + kpos + (dk & sel);
+                    vv[i] = bred32(u, negp, mu32) + c128h;
+                }
+                const uint32_t t01 = __builtin_amdgcn_perm(vv[1], vv[0], 0x05010400u);
+                const uint32_t t23 = __builtin_amdgcn_perm(vv[3], vv[2], 0x05010400u);
+                const int pl = 2 * (64 * wj + 32 * pg + c32);   // plane of digit 0 in the tile
+                uint8_t *const lo = LO + pl * RG_OPITCH + 64 * wi + 32 * rt + 8 * g + 4 * h;
+                *(uint32_t *)lo = __builtin_amdgcn_perm(t23, t01, 0x05040100u) ^ 0x80808080u;
+                *(uint32_t *)(lo + RG_OPITCH) = __builtin_amdgcn_perm(t23, t01, 0x07060302u);
+            }
+        }
+    }
+    __syncthreads();
+    // 256 planes x 128 bytes: eight lanes per plane, 16 bytes each (entries
+    // 16 j .. 16 j + 15: patch row j / 4, columns 16 (j % 4) ..)
+    const int j = lane & 7;
+    int8_t *const Rb = R + res_off(2, r0 + (j >> 2), col0 + 16 * (j & 3), kpad);
+#pragma unroll
+    for (int pass = 0; pass < 8; pass++) {
+        const int pl = 32 * pass + 8 * wave + (lane >> 3);
+        const int64_t gpl = 256 * ct + pl;   // global plane 2 q + d
+        const uint4 w4 = *(const uint4 *)(LO + pl * RG_OPITCH + 16 * j);
+        if (gpl < 2 * (int64_t)P) *(uint4 *)(Rb + gpl * plane) = w4;
+    }
+}
+
+// ------------------------------------------------------------------------
 // 3. reconstruct, in two passes.
 //    (a) recon_gemm_tile: X' = sum_q yc_q (M/p_q), with yc_q the centred
 //        y_q, as an int8 GEMM of the digits of (M/p_q) 256^s (rows = result

@@ -282,9 +282,10 @@ typedef enum {
                                      product (default) */
     MGX_PARAM_GEMM_ROWS = 5,      /* residue-GEMM tile height of two-digit plans: 0 or 128 (default), or 64
                                      (twice the workgroups, for small products) */
-    MGX_PARAM_REDUCE = 6          /* reduce kernel of two-digit plans with entries of at most 256 bytes:
-                                     0 = one workgroup per CU with the whole constant table in LDS
-                                     whenever it fits (default), 1 = entry-resident tiles of 128 entries */
+    MGX_PARAM_REDUCE = 6          /* reduce kernels of two-digit plans: 0 (default) = entries of at most
+                                     256 bytes with the whole constant table in LDS (one workgroup per CU)
+                                     whenever it fits, entries over 512 bytes as a tiled MFMA GEMM;
+                                     1 = entry-resident tiles of 128 entries, and 64-row tiles past 512 bytes */
 } mgx_param;
 int mgx_set_param(int param, int64_t value);
 int64_t mgx_get_param(int param);

@@ -35,10 +35,11 @@ def product_plan_M(primes):
     return M
 
 
-@pytest.fixture(params=[0, 1], ids=["const-resident", "tiles"])
+@pytest.fixture(params=[0, 1], ids=["fast", "tiles"])
 def reduce_kernel(request, gpu):
-    """MGX_PARAM_REDUCE: two-digit plans with entries of at most 256 bytes take the
-    constant-resident reduce (0, default) or the entry-resident tiles (1)."""
+    """MGX_PARAM_REDUCE: two-digit plans take the constant-resident reduce (entries of at
+    most 256 bytes) and the tiled MFMA GEMM (over 512 bytes) with 0, the default, and the
+    entry-resident and 64-row tiles with 1."""
     gpu.call("mgx_set_param", 6, request.param)
     yield request.param
     gpu.call("mgx_set_param", 6, 0)
@@ -46,12 +47,12 @@ def reduce_kernel(request, gpu):
 
 @pytest.mark.parametrize("transpose", [0, 1])
 # 64-, 128- and 256-byte entries (constant-resident or entry-resident), 384 bytes
-# (entry-resident, two K halves), generic (> 512 bytes)
-@pytest.mark.parametrize("bits,cap", [(200, 5), (700, 14), (2000, 34), (3000, 50), (5000, 82)])
+# (entry-resident, two K halves), generic (> 512 bytes: 5 and 9 table tiles of 128 primes)
+@pytest.mark.parametrize("bits,cap", [(200, 5), (700, 14), (2000, 34), (3000, 50), (5000, 82), (9000, 142)])
 def test_reduce_kernel(gpu, slices, reduce_kernel, transpose, bits, cap):
     mgx = gpu
-    if reduce_kernel and (slices == 4 or bits > 2048):
-        pytest.skip("the reduce kernel choice applies to two-digit plans with entries of at most 256 bytes")
+    if reduce_kernel and (slices == 4 or 2048 < bits <= 4096):
+        pytest.skip("the reduce kernel choice applies to two-digit plans outside 257-512-byte entries")
     rows, K = 70, 45
     vals = rand_ints(rows, K, bits, 3 + transpose)
     vals[0][0], vals[1][1], vals[2][2] = 0, (1 << bits) - 1, -((1 << bits) - 1)
