@@ -52,24 +52,41 @@ struct PlainJob {
 };
 
 struct Queue {
-    unsigned long long *counter;
-    const int64_t *prefix;   // njobs + 1 tile offsets
+    unsigned long long *counter;   // QRANGES counters, QPITCH apart
+    const int64_t *prefix;         // njobs + 1 tile offsets
     int njobs;
 };
 
-__device__ __forceinline__ bool next_item(const Queue &q, int &job, int64_t &tile)
+// The workgroup's claiming state (used by thread 0): the range it claims from
+// and how often it has moved on.
+struct Claim {
+    int cur, moved;
+    __device__ Claim() : cur((int)(blockIdx.x % QRANGES)), moved(0) {}
+};
+
+__device__ __forceinline__ bool next_item(const Queue &q, Claim &cl, int &job, int64_t &tile)
 {
     __shared__ int64_t s_item;
-    if (threadIdx.x == 0) s_item = (int64_t)atomicAdd(q.counter, 1ull);
+    if (threadIdx.x == 0) {
+        const int64_t total = q.prefix[q.njobs];
+        int64_t it;
+        do {
+            it = queue_claim_once([&](int x) { return atomicAdd(q.counter + QPITCH * x, 1ull); }, total, cl.cur,
+                                  cl.moved);
+        } while (it == -1);
+        s_item = it;
+    }
     __syncthreads();
     const int64_t it = s_item;
     __syncthreads();
-    return queue_map(q.prefix, q.njobs, it, &job, &tile);
+    return it >= 0 && queue_map(q.prefix, q.njobs, it, &job, &tile);
 }
 
+constexpr int NSLOTS = 32;   // queues of a batch (counter slots)
 __global__ void reset_counters_kernel(unsigned long long *c, int n)
 {
-    if ((int)threadIdx.x < n) c[threadIdx.x] = 0;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) c[i] = 0;
 }
 
 // Entry-resident and generic reduce jobs run in kernels of their own, so
@@ -83,7 +100,8 @@ __global__ __launch_bounds__(256) void batch_reduce_res_kernel(Queue q, const Re
 {
     int j;
     int64_t t;
-    while (next_item(q, j, t)) {
+    Claim cl;
+    while (next_item(q, cl, j, t)) {
         const ReduceJob &b = jobs[j];
         reduce_tile_res<S, KS>(t, b.pc, b.cst, b.P, b.nrb, b.kb, b.xbits, b.X, b.Xrs, b.W, b.transpose, b.rows, b.K,
                                b.kpad,
@@ -100,7 +118,8 @@ __global__ __launch_bounds__(256, 2) void batch_reduce_gemm_kernel(Queue q, cons
 {
     int j;
     int64_t t;
-    while (next_item(q, j, t)) {
+    Claim cl;
+    while (next_item(q, cl, j, t)) {
         const ReduceJob &b = jobs[j];
         reduce_gemm_tile(t, b.pc, b.cst, b.P, b.kb, b.xbits, b.X, b.Xrs, b.W, b.transpose, b.rows, b.K, b.kpad, b.nent,
                          b.R, b.plane, err);
@@ -112,7 +131,8 @@ __global__ __launch_bounds__(256) void batch_reduce_gen_kernel(Queue q, const Re
 {
     int j;
     int64_t t;
-    while (next_item(q, j, t)) {
+    Claim cl;
+    while (next_item(q, cl, j, t)) {
         const ReduceJob &b = jobs[j];
         reduce_tile<S>(t, b.pc, b.cst, b.P, b.nrb, b.kb, b.xbits, b.X, b.Xrs, b.W, b.transpose, b.rows, b.K, b.kpad,
                        b.nent,
@@ -126,7 +146,8 @@ __global__ __launch_bounds__(256, 1) void batch_gemm_kernel(Queue q, const GemmJ
 {
     int j;
     int64_t t;
-    while (next_item(q, j, t)) {
+    Claim cl;
+    while (next_item(q, cl, j, t)) {
         const GemmJob &g = jobs[j];
         gemm_tile<CHUNKED>(t, g.pc, g.RA, g.RB, g.planeA, g.planeB, g.kpad, g.rowsC, g.colsC, g.ntm, g.ntn, g.total,
                            g.Y, g.ystride);
@@ -139,7 +160,8 @@ __global__ __launch_bounds__(256, 2) void batch_gemm2_kernel(Queue q, const Gemm
 {
     int j;
     int64_t t;
-    while (next_item(q, j, t)) {
+    Claim cl;
+    while (next_item(q, cl, j, t)) {
         const GemmJob &g = jobs[j];
         gemm_tile2<CHUNKED, 2, 2>(t, g.pc, g.P, g.RA, g.RB, g.planeA, g.planeB, g.kpad, g.rowsC, g.colsC, g.ntm,
                                   g.ntn, g.Y, g.ystride);
@@ -152,7 +174,8 @@ __global__ __launch_bounds__(256, 2) void batch_recon_gemm_kernel(Queue q, const
 {
     int j;
     int64_t t;
-    while (next_item(q, j, t)) {
+    Claim cl;
+    while (next_item(q, cl, j, t)) {
         const ReconJob &r = jobs[j];
         recon_gemm_tile<S, false>(t, r.pc, r.P, r.Mc, r.Kr, r.nrb, r.nw, r.Y, r.ystride, r.nent, r.Wv, r.Cb, r.G);
         __syncthreads();
@@ -163,7 +186,8 @@ __global__ __launch_bounds__(256) void batch_recon_carry_kernel(Queue q, const R
 {
     int j;
     int64_t t;
-    while (next_item(q, j, t)) {
+    Claim cl;
+    while (next_item(q, cl, j, t)) {
         const ReconJob &r = jobs[j];
         recon_carry_block(t, r.Wv, r.Cb, r.ystride, r.nw, r.nrb, r.G, r.gshift, r.M32, r.nent, r.colsC, r.C, r.Crs, r.W,
                           r.cap, err);
@@ -176,7 +200,8 @@ __global__ __launch_bounds__(256) void batch_plain_kernel(Queue q, const PlainJo
 {
     int j;
     int64_t t;
-    while (next_item(q, j, t)) {
+    Claim cl;
+    while (next_item(q, cl, j, t)) {
         const PlainJob &p = jobs[j];
         plain_entries<NB, false, false>(t, p.K, p.M, p.N, p.C, p.Crs, p.A, p.Ars, p.B, p.Brs, p.W, p.cap, p.toom, err);
     }
@@ -332,7 +357,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
         o_wv[i] = L.take(v.nw * d.ystride * 4);
         o_ga[i] = L.take(ceil_div(4 * v.nw, RECON_ROWS) * d.ystride * 12);   // G (int64), then Cb (int32)
     }
-    const int64_t o_ctr = L.take(64 * 8);
+    const int64_t o_ctr = L.take(NSLOTS * QRANGES * QPITCH * 8);
     int64_t o_rj[6], o_rp[6], o_gj[4], o_gp[4], o_cj[2], o_cp[2], o_pj[NCLASS], o_pp[NCLASS];
     for (int c = 0; c < 6; c++) { o_rj[c] = L.take((int64_t)(nred[c] * sizeof(ReduceJob))); o_rp[c] = L.take((int64_t)(nred[c] + 1) * 8); }
     for (int c = 0; c < 4; c++) { o_gj[c] = L.take((int64_t)(ngem[c] * sizeof(GemmJob))); o_gp[c] = L.take((int64_t)(ngem[c] + 1) * 8); }
@@ -432,7 +457,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
     unsigned *err = device_error_word();
     auto queue = [&](int slot, int64_t o_prefix, size_t njobs) {
         Queue q;
-        q.counter = ctr + slot;
+        q.counter = ctr + (int64_t)slot * QRANGES * QPITCH;
         q.prefix = (const int64_t *)(base + o_prefix);
         q.njobs = (int)njobs;
         return q;
@@ -441,7 +466,8 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
     if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) { mgx_batch_destroy(b); return fail(MGX_ERR_HIP, "stream"); }
     hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, cs, ctr, 64);
+        hipLaunchKernelGGL(reset_counters_kernel, dim3(NSLOTS * QRANGES * QPITCH / 256), dim3(256), 0, cs, ctr,
+                           NSLOTS * QRANGES * QPITCH);
         for (int c = 0; c < NCLASS; c++) {
             if (pj[c].empty()) continue;
             Queue q = queue(9 + c, o_pp[c], pj[c].size());
@@ -531,10 +557,12 @@ int mgx_batch_queue_info(const mgx_batch *b, int64_t i, int *kind, int64_t *item
     if (items) *items = q.items;
     if (grid) *grid = q.grid;
     if (claims) {
-        unsigned long long c = 0;
+        unsigned long long c[QRANGES * QPITCH];
         MGX_HIP(hipDeviceSynchronize());
-        MGX_HIP(hipMemcpy(&c, b->counters + q.slot, sizeof c, hipMemcpyDeviceToHost));
-        *claims = (int64_t)c;
+        MGX_HIP(hipMemcpy(c, b->counters + (int64_t)q.slot * QRANGES * QPITCH, sizeof c, hipMemcpyDeviceToHost));
+        int64_t sum = 0;
+        for (int x = 0; x < QRANGES; x++) sum += (int64_t)c[QPITCH * x];
+        *claims = sum;
     }
     return MGX_OK;
 }
@@ -543,6 +571,12 @@ int mgx_queue_map(const int64_t *prefix, int njobs, int64_t item, int *job, int6
 {
     if (!prefix || !job || !tile) return fail(MGX_ERR_ARG, "null argument");
     return queue_map(prefix, njobs, item, job, tile) ? 0 : 1;
+}
+
+int64_t mgx_queue_claim(uint64_t *counters, int64_t total, int *cur, int *moved)
+{
+    if (!counters || !cur || !moved || *cur < 0 || *cur >= QRANGES) return -3;
+    return queue_claim_once([&](int x) { return counters[x]++; }, total, *cur, *moved);
 }
 
 int mgx_batch_destroy(mgx_batch *b)

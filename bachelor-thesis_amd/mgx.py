@@ -132,6 +132,7 @@ def _declare(L):
         "mgx_batch_queue_info": [c_vp, c_i64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(c_i64),
                                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(c_i64)],
         "mgx_queue_map": [c_vp, ctypes.c_int, c_i64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(c_i64)],
+        "mgx_queue_claim": [c_vp, c_i64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
@@ -160,6 +161,7 @@ def _declare(L):
     L.mgx_crt_noalloc_scratch_bytes.restype = c_i64
     L.mgx_crt_plan_slices.argtypes = [c_vp]
     L.mgx_crt_plan_slices.restype = c_i64
+    L.mgx_queue_claim.restype = c_i64
 
 
 class Product(ctypes.Structure):

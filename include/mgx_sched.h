@@ -147,11 +147,13 @@ int64_t mgx_batch_scratch_bytes(const mgx_batch *b);
 int mgx_batch_destroy(mgx_batch *b);
 
 /* Work queues of a batch, in launch order (one per persistent launch).  A
- * queue is the concatenation of its jobs' tile ranges; workgroups claim
- * items from one counter until a claim falls past the end, so after a
- * replay every queue's counter reads items + grid (claims).  Reading the
- * claims synchronizes the device.  Diagnostics with no reference counterpart
- * (the reference's host scheduler, mat-gpu/job.cu:183-336, has none). */
+ * queue is the concatenation of its jobs' tile ranges, split into 8
+ * contiguous item ranges with a counter each (one per XCD); a workgroup
+ * claims from its XCD's range first and moves on to the next range when a
+ * claim there fails, so after a replay the counters of a queue sum to
+ * items + 8 grid (claims).  Reading the claims synchronizes the device.
+ * Diagnostics with no reference counterpart (the reference's host
+ * scheduler, mat-gpu/job.cu:183-336, has none). */
 typedef enum {
     MGX_QUEUE_PLAIN = 0, MGX_QUEUE_REDUCE = 1, MGX_QUEUE_RESIDUE_GEMM = 2, MGX_QUEUE_RECONSTRUCT = 3,
     MGX_QUEUE_CARRY = 4
@@ -162,6 +164,11 @@ int mgx_batch_queue_info(const mgx_batch *b, int64_t i, int *kind, int64_t *item
  * tile) for prefix[0] = 0 <= ... <= prefix[njobs] = items.  Returns 0, or 1
  * when item is past the end (the claim that ends a workgroup's loop). */
 int mgx_queue_map(const int64_t *prefix, int njobs, int64_t item, int *job, int64_t *tile);
+/* One claim attempt (host copy of the device code, counters[0..7] the
+ * ranges' counters): the claimed item; -1 when range *cur is drained (*cur
+ * and *moved advance); -2 when all 8 ranges have been found drained; -3 on
+ * bad arguments.  A workgroup starts with *cur = its XCD, *moved = 0. */
+int64_t mgx_queue_claim(uint64_t *counters, int64_t total, int *cur, int *moved);
 
 #ifdef __cplusplus
 }
