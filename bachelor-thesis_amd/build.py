@@ -19,7 +19,8 @@ LIB = os.path.join(PKG, "lib", "libmgx.so")
 BIN = os.path.join(PKG, "bin", "mat-gpu")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["api.cpp", "crt.hip", "plain.hip", "rec.cpp", "crt_tree.cpp", "batch.hip", "mlp.hip", "tune.hip", "peaks.hip"]
+SOURCES = ["api.cpp", "crt.hip", "plain.hip", "rec.cpp", "crt_tree.cpp", "crt_tree_dev.hip", "batch.hip", "mlp.hip",
+           "tune.hip", "peaks.hip"]
 # Device code: gfx950 only; packed-FP32 instructions disabled (the cost
 # model's float code must not be turned into v_pk_*_f32 forms).
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-Wall", "-Wno-unused-function",

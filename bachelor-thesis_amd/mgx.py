@@ -133,6 +133,8 @@ def _declare(L):
                                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(c_i64)],
         "mgx_queue_map": [c_vp, ctypes.c_int, c_i64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(c_i64)],
         "mgx_queue_claim": [c_vp, c_i64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+        "mgx_crt_split_list_device": [c_vp, c_vp, c_i64, c_vp, c_vp, ctypes.c_int, c_i64, c_vp],
+        "mgx_crt_merge_list_device": [c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, ctypes.c_int, c_i64, c_vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

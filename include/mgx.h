@@ -254,7 +254,7 @@ int64_t mgx_crt_y_stride(int64_t K, int64_t N);
  * 256 unpinned ones are kept.  Housekeeping with no reference counterpart. */
 int mgx_crt_plan_cache_stats(int64_t *total, int64_t *pinned);
 
-/* ---------------- kept reference CRT API (mat-gpu/gpu_crt.h:6-14; host) ---------------- */
+/* ---------------- kept reference CRT API (mat-gpu/gpu_crt.h:6-14; host, and a device batch form) ---------------- */
 /* Same arguments and meaning as gpu_crt_* on entries of capacity cap, using
  * this library's own arithmetic (no GMP).  make_split_tree allocates the
  * four arrays (cap+1 words per entry) with malloc; the caller frees them
@@ -269,6 +269,16 @@ int mgx_crt_split_list(mgx_limb *res, const mgx_limb *x, const mgx_limb *ms, con
                        int64_t cap);
 int mgx_crt_merge_list(mgx_limb *x, const mgx_limb *split, const mgx_limb *ms, const mgx_limb *ns,
                        const mgx_limb *mfs, const mgx_limb *nfs, int length, int64_t cap);
+/* gpu_crt_split_list / gpu_crt_merge_list (mat-gpu/gpu_crt.h:10-12, crt.cpp:126-164) on the device, for
+ * count entries at once through one shared tree: X holds count entries, res (split) count * 2^length
+ * leaves (entry e's leaves at e * 2^length .. , in the reference's order); the tree arrays (length
+ * entries each, as from mgx_crt_make_split_tree) are device copies.  All cap+1 words per entry; device
+ * pointers; synchronous (MGX_ERR_CAPACITY when a value exceeds cap). */
+int mgx_crt_split_list_device(mgx_limb *res, const mgx_limb *X, int64_t count, const mgx_limb *ms,
+                              const mgx_limb *ns, int length, int64_t cap, void *stream);
+int mgx_crt_merge_list_device(mgx_limb *X, const mgx_limb *split, int64_t count, const mgx_limb *ms,
+                              const mgx_limb *ns, const mgx_limb *mfs, const mgx_limb *nfs, int length, int64_t cap,
+                              void *stream);
 
 /* ---------------- tunables ---------------- */
 typedef enum {
