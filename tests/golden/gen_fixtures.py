@@ -127,11 +127,49 @@ def text_case():
     return gen, out
 
 
+def text_cases():
+    """More -g/-p pairs: wide entries (3000 bits, with all-ones extremes and
+    zeros), a zero row and a zero column, a 1 x 1 result, and an empty inner
+    dimension (C is all zeros)."""
+    rng = random.Random(11)
+    cases = []
+
+    def render(K, M, N, A, B):
+        gen = "%d %d %d\n" % (K, M, N)
+        for mat in (A, B):
+            for r in mat:
+                gen += " ".join(hexs(v) for v in r) + "\n"
+            gen += "\n"
+        C = matmul(A, B) if M else [[0] * N for _ in range(K)]
+        out = "".join(" ".join(hexs(v) for v in r) + "\n" for r in C) + "\n"
+        return {"name": "%dx%dx%d" % (K, M, N), "input": gen, "product": out}
+
+    K, M, N, b = 3, 5, 2, 3000
+    A = [[rand_entry(rng, b, "rand") for _ in range(M)] for _ in range(K)]
+    B = [[rand_entry(rng, b, "rand") for _ in range(N)] for _ in range(M)]
+    A[0][1], A[2][4], B[3][0] = (1 << b) - 1, -((1 << b) - 1), 0
+    cases.append(render(K, M, N, A, B))
+    K, M, N = 4, 3, 3
+    A = [[rand_entry(rng, 200, "rand") for _ in range(M)] for _ in range(K)]
+    B = [[rand_entry(rng, 200, "rand") for _ in range(N)] for _ in range(M)]
+    A[2] = [0] * M
+    for r in B:
+        r[1] = 0
+    cases.append(render(K, M, N, A, B))
+    A = [[rand_entry(rng, 700, "rand") for _ in range(7)]]
+    B = [[rand_entry(rng, 700, "rand")] for _ in range(7)]
+    cases.append(render(1, 7, 1, A, B))
+    cases.append(render(3, 0, 2, [[] for _ in range(3)], []))
+    return cases
+
+
 def main():
     with open(os.path.join(HERE, "kat_matmul.json"), "w") as f:
         json.dump(matmul_cases(), f)
     with open(os.path.join(HERE, "kat_crt.json"), "w") as f:
         json.dump(crt_vectors(), f)
+    with open(os.path.join(HERE, "kat_text_cases.json"), "w") as f:
+        json.dump(text_cases(), f)
     gen, out = text_case()
     with open(os.path.join(HERE, "kat_text_input.txt"), "w") as f:
         f.write(gen)

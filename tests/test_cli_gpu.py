@@ -24,6 +24,23 @@ def test_text_roundtrip(method):
     assert "Time taken:" in r.stderr
 
 
+def _text_cases():
+    import json
+    with open(os.path.join(GOLDEN, "kat_text_cases.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("method", ["1", "35", "8", "0"])
+@pytest.mark.parametrize("case", _text_cases(), ids=lambda c: c["name"])
+def test_text_cases(case, method):
+    """Wide (3000-bit) entries with all-ones extremes, a zero row and column, a
+    1 x 1 result and an empty inner dimension, through every route."""
+    r = subprocess.run([BIN, "-i", "-p", "-m", method], input=case["input"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout == case["product"]
+
+
 def test_generated_product_and_json():
     r = subprocess.run([BIN, "-n", "3,4,5", "-b", "300", "-m", "8", "-s", "9", "-p", "-j"], capture_output=True,
                        text=True, timeout=120)
