@@ -19,7 +19,7 @@ namespace mgx {
 
 struct ReduceJob {
     const PrimeConst *pc;
-    const int8_t *cst;
+    const int8_t *cst, *cstb;   // cstb: the table in 1 KB blocks (tiled reduce)
     int P, nrb, kb, xbits, transpose, resident;
     const uint64_t *X;
     int64_t Xrs, W, rows, K, kpad, nent, plane;
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(256, 2) void batch_reduce_gemm_kernel(Queue q, cons
     Claim cl;
     while (next_item(q, cl, j, t)) {
         const ReduceJob &b = jobs[j];
-        reduce_gemm_tile(t, b.pc, b.cst, b.P, b.kb, b.xbits, b.X, b.Xrs, b.W, b.transpose, b.rows, b.K, b.kpad, b.nent,
+        reduce_gemm_tile(t, b.pc, b.cstb, b.P, b.kb, b.xbits, b.X, b.Xrs, b.W, b.transpose, b.rows, b.K, b.kpad, b.nent,
                          b.R, b.plane, err);
     }
 }
@@ -382,7 +382,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             const bool resident = kind != 1;
             for (int t = 0; t < 2; t++) {
                 ReduceJob r;
-                r.pc = v.pc; r.cst = v.cst; r.P = (int)v.P; r.nrb = nrb; r.kb = (int)v.kb; r.transpose = t;
+                r.pc = v.pc; r.cst = v.cst; r.cstb = v.cstb; r.P = (int)v.P; r.nrb = nrb; r.kb = (int)v.kb; r.transpose = t;
                 r.xbits = (int)(t ? v.bitsB : v.bitsA);
                 r.resident = resident;
                 r.X = (const uint64_t *)(t ? p.B : p.A); r.Xrs = t ? p.Brs : p.Ars; r.W = p.cap + 1;

@@ -128,14 +128,14 @@ __global__ __launch_bounds__(256) void crt_reduce_res_kernel(
 // in XCD-aware order: the tiles an XCD runs together share entry and table
 // tiles in its L2.
 __global__ __launch_bounds__(256, 2) void crt_reduce_gemm_kernel(
-    const PrimeConst *__restrict__ pc, const int8_t *__restrict__ cst, int P, int kb, int xbits,
+    const PrimeConst *__restrict__ pc, const int8_t *__restrict__ cstb, int P, int kb, int xbits,
     const uint64_t *__restrict__ X, int64_t Xrs, int64_t W, int transpose, int64_t rows, int64_t K, int64_t kpad,
     int64_t nent, int8_t *__restrict__ R, int64_t plane, int64_t total, unsigned *__restrict__ err)
 {
     const int64_t per = (total + 7) / 8;
     const int64_t v = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
     if (v >= total) return;
-    reduce_gemm_tile(v, pc, cst, P, kb, xbits, X, Xrs, W, transpose, rows, K, kpad, nent, R, plane, err);
+    reduce_gemm_tile(v, pc, cstb, P, kb, xbits, X, Xrs, W, transpose, rows, K, kpad, nent, R, plane, err);
 }
 // The constant-resident reduce: one workgroup per CU, units [0, nu0) of 256
 // entries (4 x 64 patches) from side 0, the rest from side 1.
@@ -219,6 +219,7 @@ struct mgx_crt_plan {
     std::vector<uint32_t> primes;
     PrimeConst *d_pc = nullptr;
     int8_t *d_cst = nullptr;
+    int8_t *d_cstb = nullptr;   // the reduce table in 1 KB blocks (rg_table_blocked), when the tiled reduce takes it
     int8_t *d_mc = nullptr;
     uint32_t *d_m32 = nullptr;
 };
@@ -433,11 +434,14 @@ int plan_create_slices(mgx_crt_plan **out, int64_t K, int64_t bitsA, int64_t bit
     }
     std::vector<uint32_t> m32(pl->Lbp / 4, 0);
     for (size_t i = 0; i < M.size() && i < m32.size(); i++) m32[i] = M[i];
+    std::vector<int8_t> cstb;
+    if (S == 2 && pl->kb > RES_KB) cstb = rg_table_blocked(cst.data(), P, pl->kb);
 
     hipError_t e;
     if ((e = hipMalloc(&pl->d_pc, sizeof(PrimeConst) * P)) != hipSuccess ||
         (e = hipMalloc(&pl->d_cst, cst.size())) != hipSuccess || (e = hipMalloc(&pl->d_mc, mc.size())) != hipSuccess ||
-        (e = hipMalloc(&pl->d_m32, m32.size() * 4)) != hipSuccess) {
+        (e = hipMalloc(&pl->d_m32, m32.size() * 4)) != hipSuccess ||
+        (!cstb.empty() && (e = hipMalloc(&pl->d_cstb, cstb.size())) != hipSuccess)) {
         mgx_crt_plan_destroy(pl);
         return fail(MGX_ERR_OOM, "plan allocation: %s", hipGetErrorString(e));
     }
@@ -445,6 +449,7 @@ int plan_create_slices(mgx_crt_plan **out, int64_t K, int64_t bitsA, int64_t bit
     MGX_HIP(hipMemcpy(pl->d_cst, cst.data(), cst.size(), hipMemcpyHostToDevice));
     MGX_HIP(hipMemcpy(pl->d_mc, mc.data(), mc.size(), hipMemcpyHostToDevice));
     MGX_HIP(hipMemcpy(pl->d_m32, m32.data(), m32.size() * 4, hipMemcpyHostToDevice));
+    if (!cstb.empty()) MGX_HIP(hipMemcpy(pl->d_cstb, cstb.data(), cstb.size(), hipMemcpyHostToDevice));
     *out = pl;
     return MGX_OK;
 }
@@ -505,6 +510,7 @@ int mgx_crt_plan_destroy(mgx_crt_plan *pl)
     if (!pl) return MGX_OK;
     hipFree(pl->d_pc);
     hipFree(pl->d_cst);
+    hipFree(pl->d_cstb);
     hipFree(pl->d_mc);
     hipFree(pl->d_m32);
     delete pl;
@@ -552,7 +558,7 @@ int mgx_crt_reduce(const mgx_crt_plan *pl, int transpose, int64_t rows, int64_t 
         const int64_t total = rg_tiles(nent, pl->P), grid = round_up(total, 8);
         if (grid > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "matrix too large for one reduce launch");
         hipLaunchKernelGGL(crt_reduce_gemm_kernel, dim3((unsigned)grid), dim3(256), 0, as_stream(stream), pl->d_pc,
-                           pl->d_cst, (int)pl->P, (int)pl->kb, xbits, (const uint64_t *)X, Xrs, cap + 1, transpose, rows,
+                           pl->d_cstb, (int)pl->P, (int)pl->kb, xbits, (const uint64_t *)X, Xrs, cap + 1, transpose, rows,
                            K, kpad, nent, R, plane, total, device_error_word());
     } else
         hipLaunchKernelGGL(gen, dim3((unsigned)nblk), dim3(256), 0, as_stream(stream), pl->d_pc, pl->d_cst,
@@ -852,7 +858,7 @@ __global__ void zero_entries_kernel(uint64_t *C, int64_t Crs, int64_t K, int64_t
 
 int plan_view(const mgx_crt_plan *pl, PlanView *v)
 {
-    v->pc = pl->d_pc; v->cst = pl->d_cst; v->mc = pl->d_mc; v->m32 = pl->d_m32;
+    v->pc = pl->d_pc; v->cst = pl->d_cst; v->cstb = pl->d_cstb; v->mc = pl->d_mc; v->m32 = pl->d_m32;
     v->S = pl->S; v->P = pl->P; v->kb = pl->kb; v->Kr = pl->Kr; v->rows4pad = pl->rows4pad; v->Lbp = pl->Lbp;
     v->out_limbs = pl->out_limbs;
     v->nw = pl->nw;

@@ -11,6 +11,8 @@
 // prime w (S = 4) or primes 2w and 2w+1 in its low and high halves (S = 2).
 #pragma once
 #include "common.h"
+#include <cstring>
+#include <vector>
 
 namespace mgx {
 
@@ -1053,7 +1055,11 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
 //     RG_STAGES-deep LDS ring; the entries' 16-byte chunks come straight from
 //     their limbs (per-lane 64-bit addresses, kept inside the entry) and are
 //     masked at |size| and biased (byte - 128, or 127 - byte for a negative
-//     entry) when a wave reads its fragments.  Stage layout: [entries 128 x
+//     entry) when a wave reads its fragments.  The table is read from a
+//     copy in 1 KB blocks (rg_table_blocked), so that each table DMA
+//     instruction reads one contiguous kilobyte: rows kb bytes apart (a power
+//     of two) put a DMA's sixteen rows on few memory channels, which measured
+//     1.5x slower for the whole kernel.  Stage layout: [entries 128 x
 //     64 B][table 256 x 64 B], chunk c of row r at slot c ^ g2_swz(r).  The
 //     epilogue is the constant-resident reduce's with the digit-1 sums
 //     reduced first (|acc| <= kb 2^14 < 2^27 for the widths two-digit plans
@@ -1068,6 +1074,25 @@ constexpr int RG_OPITCH = 144;                  // bytes per plane of the output
 constexpr int RG_GE = 8;                        // entry tiles per group
 static_assert(RG_STAGES * RG_STAGE >= 256 * RG_OPITCH, "the output tile reuses the ring");
 inline int64_t rg_tiles(int64_t nent, int64_t P) { return ceil_div(nent, 128) * ceil_div(P, 128); }
+// The reduce table (rows S q + s of kb bytes, S = 2) in 1 KB blocks: block
+// (qb, d, c) at ((2 qb + d) (kb / 64) + c) KB holds bytes 64 c .. 64 c + 63 of
+// table row 2 q + d for the primes q = 16 qb .. 16 qb + 15 (one 64-byte row
+// each, zero past the last prime); 16 qb runs to P rounded up to 128, the
+// primes a table tile spans.
+inline std::vector<int8_t> rg_table_blocked(const int8_t *cst, int64_t P, int64_t kb)
+{
+    const int64_t nqb = ceil_div(P, 128) * 8, nc = kb / 64;
+    std::vector<int8_t> out((size_t)(nqb * 2 * nc * 1024), 0);
+    for (int64_t qb = 0; qb < nqb; qb++)
+        for (int64_t d = 0; d < 2; d++)
+            for (int64_t c = 0; c < nc; c++)
+                for (int64_t i = 0; i < 16; i++) {
+                    const int64_t q = 16 * qb + i;
+                    if (q < P)
+                        memcpy(out.data() + ((2 * qb + d) * nc + c) * 1024 + 64 * i, cst + (2 * q + d) * kb + 64 * c, 64);
+                }
+    return out;
+}
 
 // 16 bytes from the lane's own address to LDS at lds_base + 16 * lane
 __device__ __forceinline__ void dma16v(const void *addr, uint32_t lds_base)
@@ -1076,7 +1101,7 @@ __device__ __forceinline__ void dma16v(const void *addr, uint32_t lds_base)
 }
 
 __device__ __forceinline__ void reduce_gemm_tile(int64_t v, const PrimeConst *__restrict__ pc,
-    const int8_t *__restrict__ cst, int P, int kb, int xbits, const uint64_t *__restrict__ X, int64_t Xrs, int64_t W,
+    const int8_t *__restrict__ cstb, int P, int kb, int xbits, const uint64_t *__restrict__ X, int64_t Xrs, int64_t W,
     int transpose, int64_t rows, int64_t K, int64_t kpad, int64_t nent, int8_t *__restrict__ R, int64_t plane,
     unsigned *__restrict__ err)
 {
@@ -1110,7 +1135,7 @@ __device__ __forceinline__ void reduce_gemm_tile(int64_t v, const PrimeConst *__
     // row + l / 4, slot l % 4, i.e. chunk (l % 4) ^ g2_swz(row) = (l % 4) ^ ((l >> 4) & 3)
     const int chunk = (lane & 3) ^ ((lane >> 4) & 3);
     const int8_t *eaddr[2];
-    uint32_t toff[4];
+    const int8_t *tbase[4];   // table rows 64 wave + 16 u ..: block (qb, d) = (8 ct + 2 wave + u % 2, u / 2)
 #pragma unroll
     for (int u = 0; u < 2; u++) {
         int64_t b;
@@ -1120,11 +1145,9 @@ __device__ __forceinline__ void reduce_gemm_tile(int64_t v, const PrimeConst *__
         eaddr[u] = (const int8_t *)(X + b) + 16 * chunk;
     }
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-        const int row = 64 * wave + 16 * u + (lane >> 2);
-        const int64_t q = 128 * ct + 32 * (row >> 6) + (row & 31), d = (row >> 5) & 1;
-        toff[u] = (uint32_t)((q < P ? 2 * q + d : 0) * kb + 16 * chunk);
-    }
+    for (int u = 0; u < 4; u++)
+        tbase[u] = uniform_ptr(cstb + ((2 * (8 * ct + 2 * wave_u + (u & 1)) + (u >> 1)) * (int64_t)(kb / 64)) * 1024);
+    const uint32_t tvoff = (uint32_t)(64 * (lane >> 2) + 16 * chunk);
     // bytes of an entry at offsets >= emax come from its last 16 bytes (masked)
     const int64_t emax = 8 * W - 16;
 #define ISSUE(k_)                                                                                       \
@@ -1134,9 +1157,9 @@ __device__ __forceinline__ void reduce_gemm_tile(int64_t v, const PrimeConst *__
         _Pragma("unroll") for (int u = 0; u < 2; u++)                                                   \
             dma16v(eaddr[u] + min<int64_t>(ko_, emax - 16 * chunk),                                     \
                    __builtin_amdgcn_readfirstlane(sb_ + (32 * wave_u + 16 * u) * 64));                  \
-        const int8_t *tb_ = uniform_ptr(cst + ko_);                                                     \
         _Pragma("unroll") for (int u = 0; u < 4; u++)                                                   \
-            dma16(tb_, toff[u], __builtin_amdgcn_readfirstlane(sb_ + 128 * 64 + (64 * wave_u + 16 * u) * 64)); \
+            dma16(tbase[u] + (k_) * 1024, tvoff,                                                        \
+                  __builtin_amdgcn_readfirstlane(sb_ + 128 * 64 + (64 * wave_u + 16 * u) * 64));         \
     }
 
     // the fragments' entries (rows 64 wi + 32 rt + c32): limbs below |size| and sign
@@ -1168,31 +1191,42 @@ __device__ __forceinline__ void reduce_gemm_tile(int64_t v, const PrimeConst *__
         __syncthreads();
         if (k + RG_STAGES - 1 < nsteps) { ISSUE(k + RG_STAGES - 1); }
         const int8_t *se = LS + (k % RG_STAGES) * RG_STAGE, *st = se + 128 * 64;
+        // every fragment of the step is read before the first MFMA, so that
+        // the second half's LDS reads run under the first half's MFMAs
+        v4i raw[2][2], bfr[2][4];
 #pragma unroll
         for (int kk = 0; kk < 2; kk++) {
             const int ch = 2 * kk + h;
-            v4i af[2], bfr[4];
 #pragma unroll
             for (int rt = 0; rt < 2; rt++) {
                 const int row = 64 * wi + 32 * rt + c32;
-                const v4i raw = *(const v4i *)(se + row * 64 + 16 * (ch ^ g2_swz(row)));
-                // limbs 8k + 4kk + 2h and + 1 of the entry; masked at |size|, then biased
-                const int l0 = 8 * k + 4 * kk + 2 * h;
-                const uint32_t flip = fneg[rt] ? 0x7f7f7f7fu : 0x80808080u;
-                af[rt] = (v4i){(int)((l0 < fnl[rt] ? (uint32_t)raw[0] : 0u) ^ flip),
-                               (int)((l0 < fnl[rt] ? (uint32_t)raw[1] : 0u) ^ flip),
-                               (int)((l0 + 1 < fnl[rt] ? (uint32_t)raw[2] : 0u) ^ flip),
-                               (int)((l0 + 1 < fnl[rt] ? (uint32_t)raw[3] : 0u) ^ flip)};
+                raw[kk][rt] = *(const v4i *)(se + row * 64 + 16 * (ch ^ g2_swz(row)));
             }
 #pragma unroll
             for (int c = 0; c < 4; c++) {
                 const int row = 128 * wj + 32 * c + c32;
-                bfr[c] = *(const v4i *)(st + row * 64 + 16 * (ch ^ g2_swz(row)));
+                bfr[kk][c] = *(const v4i *)(st + row * 64 + 16 * (ch ^ g2_swz(row)));
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int kk = 0; kk < 2; kk++) {
+            v4i af[2];
+#pragma unroll
+            for (int rt = 0; rt < 2; rt++) {
+                // limbs 8k + 4kk + 2h and + 1 of the entry; masked at |size|, then biased
+                const int l0 = 8 * k + 4 * kk + 2 * h;
+                const uint32_t flip = fneg[rt] ? 0x7f7f7f7fu : 0x80808080u;
+                const v4i &r_ = raw[kk][rt];
+                af[rt] = (v4i){(int)((l0 < fnl[rt] ? (uint32_t)r_[0] : 0u) ^ flip),
+                               (int)((l0 < fnl[rt] ? (uint32_t)r_[1] : 0u) ^ flip),
+                               (int)((l0 + 1 < fnl[rt] ? (uint32_t)r_[2] : 0u) ^ flip),
+                               (int)((l0 + 1 < fnl[rt] ? (uint32_t)r_[3] : 0u) ^ flip)};
             }
 #pragma unroll
             for (int rt = 0; rt < 2; rt++)
 #pragma unroll
-                for (int c = 0; c < 4; c++) acc[rt][c] = mfma32(af[rt], bfr[c], acc[rt][c]);
+                for (int c = 0; c < 4; c++) acc[rt][c] = mfma32(af[rt], bfr[kk][c], acc[rt][c]);
         }
     }
 #undef ISSUE
@@ -1628,6 +1662,7 @@ inline int64_t crt_ystride(int64_t nent) { return round_up(nent < 1 ? 1 : nent, 
 struct PlanView {
     const PrimeConst *pc;
     const int8_t *cst, *mc;
+    const int8_t *cstb;   // the table in 1 KB blocks for the tiled reduce (two-digit plans, kb > RES_KB), else null
     const uint32_t *m32;
     int64_t S, P, kb, Kr, rows4pad, Lbp, out_limbs, nw;
     int64_t bitsA, bitsB;   // the operands' bit bounds (|A| < 2^bitsA, |B| < 2^bitsB)
