@@ -1418,10 +1418,111 @@ __device__ __forceinline__ int64_t gamma_term(uint32_t wd, int sub, int q, int P
     return q < P ? (int64_t)(((uint64_t)y * c_.fx) >> 31) - (yc < 0 ? (1ll << 32) : 0) : 0;
 }
 
-template <int S, bool RESIDENT>
-__device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *__restrict__ pc, int P,
+// Streaming reconstruct tile (S = 2, row blocks of Mc too large for LDS): the
+// same 128-byte x 256-entry tile and epilogue as recon_gemm_tile, with both
+// operands of every 64-byte K step brought into a ring of RS_STAGES LDS
+// stages by global_load_lds_dwordx4, two steps ahead, one barrier per step.
+// Stage layout: Mc rows 128 x 64 bytes (chunk c of row r at slot c ^
+// g2_swz(r)), then the step's 16 word planes of Y, 256 entries each, RS_YPITCH
+// bytes apart so that planes 4 apart start 32 banks apart (a lane's four words
+// of one fragment come from planes 8 kk + 4 h + i: the two half-waves' reads
+// fall in different banks).  Per wave and stage: two Mc DMAs (rows 16 (2 wave
+// + u) ..) and four Y DMAs (planes 4 wave + u); the per-prime gamma
+// constants come through the scalar cache, so that the only vector memory
+// operations in the loop are the DMAs the waits count.
+constexpr int RS_STAGES = 3;
+constexpr int RS_YPITCH = 1056;
+constexpr int RS_STAGE = 128 * 64 + 16 * RS_YPITCH;
+__device__ __forceinline__ void recon_stream_tile(int64_t blk, const PrimeConst *__restrict__ pc, int P,
     const int8_t *__restrict__ Mc, int Kr, int nrb, int nw, const uint32_t *__restrict__ Y, int64_t ystride,
-    int64_t nent, uint32_t *__restrict__ Wv, int32_t *__restrict__ Cb, int64_t *__restrict__ G, int epw = 1)
+    int64_t nent, uint32_t *__restrict__ Wv, int32_t *__restrict__ Cb, int64_t *__restrict__ G)
+{
+    constexpr int PER = 6;
+    __shared__ __attribute__((aligned(1024))) int8_t LS[RS_STAGES * RS_STAGE];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, h = lane >> 5, c32 = lane & 31;
+    const int wave_u = __builtin_amdgcn_readfirstlane(wave);
+    const int rb = (int)(blk % nrb);
+    const int64_t e0 = (blk / nrb) * RECON_ENT;
+    const int nsteps = Kr / 64, nyp = (2 * P + 3) / 4;
+    const uint32_t ls0 = __builtin_amdgcn_readfirstlane(lds_offset(LS));
+    // Mc: lane l takes row 16 (2 wave + u) + l / 4, slot l % 4, i.e. chunk (l % 4) ^ ((l >> 4) & 3)
+    const uint32_t mvoff = (uint32_t)((lane >> 2) * Kr + 16 * ((lane & 3) ^ ((lane >> 4) & 3)));
+    const int8_t *mbase[2];
+#pragma unroll
+    for (int u = 0; u < 2; u++) mbase[u] = uniform_ptr(Mc + (int64_t)(rb * RECON_ROWS + 16 * (2 * wave_u + u)) * Kr);
+    const int8_t *ybase = uniform_ptr((const int8_t *)(Y + e0));
+#define ISSUE(k_)                                                                                      \
+    {                                                                                                  \
+        const uint32_t sb_ = ls0 + ((k_) % RS_STAGES) * RS_STAGE;                                      \
+        _Pragma("unroll") for (int u = 0; u < 2; u++)                                                  \
+            dma16(mbase[u] + (k_) * 64, mvoff, __builtin_amdgcn_readfirstlane(sb_ + 16 * (2 * wave_u + u) * 64)); \
+        _Pragma("unroll") for (int u = 0; u < 4; u++) {                                                \
+            const int gp_ = min(16 * (k_) + 4 * wave_u + u, nyp - 1);                                  \
+            dma16(ybase + (int64_t)gp_ * ystride * 4, (uint32_t)(16 * lane),                           \
+                  __builtin_amdgcn_readfirstlane(sb_ + 128 * 64 + (4 * wave_u + u) * RS_YPITCH));     \
+        }                                                                                              \
+    }
+
+    v16i acc[4][2];
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+        for (int ct = 0; ct < 2; ct++) acc[rt][ct] = (v16i){0};
+    int64_t gsum[2] = {0, 0};
+#pragma unroll
+    for (int k = 0; k < RS_STAGES - 1; k++)
+        if (k < nsteps) { ISSUE(k); }
+    for (int k = 0; k < nsteps; k++) {
+        wait_stages<PER>(min(nsteps - 1, k + RS_STAGES - 2) - k);
+        __syncthreads();
+        if (k + RS_STAGES - 1 < nsteps) { ISSUE(k + RS_STAGES - 1); }
+        const int8_t *sa = LS + (k % RS_STAGES) * RS_STAGE;
+        const uint32_t *sy = (const uint32_t *)(sa + 128 * 64);
+        v4i af[2][4], yf[2][2];
+#pragma unroll
+        for (int kk = 0; kk < 2; kk++) {
+#pragma unroll
+            for (int ct = 0; ct < 2; ct++)
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    yf[ct][kk][i] = (int)sy[(8 * kk + 4 * h + i) * (RS_YPITCH / 4) + 64 * wave + 32 * ct + c32];
+#pragma unroll
+            for (int rt = 0; rt < 4; rt++) {
+                const int row = rt * 32 + c32;
+                af[kk][rt] = *(const v4i *)(sa + row * 64 + 16 * ((2 * kk + h) ^ g2_swz(row)));
+            }
+        }
+        if (k % nrb == rb) {   // this row block's share of the gamma sums
+#pragma unroll
+            for (int kk = 0; kk < 2; kk++)
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int sub = 0; sub < 2; sub++) {
+                        // both half-waves' primes from wave-uniform addresses; none past the last
+                        const int q0 = 2 * (16 * k + 8 * kk + i) + sub, q1 = q0 + 8;
+                        const int32_t f0 = q0 < P ? pc[min(q0, P - 1)].f45 : 0;
+                        const int32_t f1 = q1 < P ? pc[min(q1, P - 1)].f45 : 0;
+                        const int32_t f = h ? f1 : f0;
+#pragma unroll
+                        for (int ct = 0; ct < 2; ct++) gsum[ct] += gamma_term<2>((uint32_t)yf[ct][kk][i], sub, 0, P, pc, f);
+                    }
+        }
+#pragma unroll
+        for (int kk = 0; kk < 2; kk++)
+#pragma unroll
+            for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+                for (int ct = 0; ct < 2; ct++) acc[rt][ct] = mfma32(af[kk][rt], yf[ct][kk], acc[rt][ct]);
+    }
+#undef ISSUE
+    recon_store(acc, gsum, e0, wave, h, c32, rb, nw, nrb, nent, Wv, Cb, G);
+}
+
+template <int S, bool RESIDENT>
+__device__ __forceinline__ void recon_gemm_tile_regs(int64_t blk, const PrimeConst *__restrict__ pc, int P,
+    const int8_t *__restrict__ Mc, int Kr, int nrb, int nw, const uint32_t *__restrict__ Y, int64_t ystride,
+    int64_t nent, uint32_t *__restrict__ Wv, int32_t *__restrict__ Cb, int64_t *__restrict__ G, int epw)
 {
     __shared__ __attribute__((aligned(16))) int8_t LAs[RESIDENT ? 16 : RECON_ROWS * LDS_PITCH];
     extern __shared__ __attribute__((aligned(16))) int8_t LDYN[];
@@ -1554,6 +1655,20 @@ __device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *_
 #undef YLOAD
     recon_store(acc, gsum, e0, wave, h, c32, rb, nw, nrb, nent, Wv, Cb, G);
     }   // entry blocks
+}
+
+// One reconstruct tile: the streaming tile for two-digit plans whose row
+// blocks do not fit LDS, else the register-fed tile above (RESIDENT: the row
+// block sits in LDS for epw entry blocks).
+template <int S, bool RESIDENT>
+__device__ __forceinline__ void recon_gemm_tile(int64_t blk, const PrimeConst *__restrict__ pc, int P,
+    const int8_t *__restrict__ Mc, int Kr, int nrb, int nw, const uint32_t *__restrict__ Y, int64_t ystride,
+    int64_t nent, uint32_t *__restrict__ Wv, int32_t *__restrict__ Cb, int64_t *__restrict__ G, int epw = 1)
+{
+    if constexpr (S == 2 && !RESIDENT)
+        recon_stream_tile(blk, pc, P, Mc, Kr, nrb, nw, Y, ystride, nent, Wv, Cb, G);
+    else
+        recon_gemm_tile_regs<S, RESIDENT>(blk, pc, P, Mc, Kr, nrb, nw, Y, ystride, nent, Wv, Cb, G, epw);
 }
 
 constexpr int CARRY_ENT = 256;   // entries per workgroup
