@@ -207,8 +207,9 @@ const char *mgx_version(void) { return "mgx 0.1 gfx950"; }
 
 int mgx_set_param(int which, int64_t value)
 {
-    if (which < 0 || which > MGX_PARAM_REDUCE) return fail(MGX_ERR_ARG, "unknown parameter %d", which);
+    if (which < 0 || which > MGX_PARAM_RECON) return fail(MGX_ERR_ARG, "unknown parameter %d", which);
     if (which == MGX_PARAM_REDUCE && value != 0 && value != 1) return fail(MGX_ERR_ARG, "reduce kernel must be 0 or 1");
+    if (which == MGX_PARAM_RECON && (value < 0 || value > 2)) return fail(MGX_ERR_ARG, "reconstruct mode must be 0, 1 or 2");
     if (which == MGX_PARAM_GEMM_ROWS && value != 0 && value != 64 && value != 128)
         return fail(MGX_ERR_ARG, "GEMM tile rows must be 0, 64 or 128");
     if (which == MGX_PARAM_TRANSFER && value < 0) return fail(MGX_ERR_ARG, "transfer size must be >= 0");

@@ -195,6 +195,19 @@ __global__ __launch_bounds__(256, 2) void crt_recon_gemm_kernel(
     recon_gemm_tile<S, RESIDENT>(v, pc, P, Mc, Kr, nrb, nw, Y, ystride, nent, Wv, Cb, G, epw);
 }
 
+// One-pass reconstruct of two-digit plans: one workgroup per 256 entries, in
+// XCD-aware order (neighbouring entry blocks share Mc and Y lines in an L2)
+__global__ __launch_bounds__(256, 2) void crt_recon_fused_kernel(
+    const PrimeConst *__restrict__ pc, int P, const int8_t *__restrict__ Mc, int Kr, int nrb, int nw,
+    const uint32_t *__restrict__ Y, int64_t ystride, int64_t nent, const uint32_t *__restrict__ M32, int64_t colsC,
+    uint64_t *__restrict__ C, int64_t Crs, int64_t W, int64_t cap, unsigned *__restrict__ err, int64_t nblk)
+{
+    const int64_t per = (nblk + 7) / 8;
+    const int64_t v = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (v >= nblk) return;
+    recon_fused_block(v, pc, P, Mc, Kr, nrb, nw, Y, ystride, nent, M32, colsC, C, Crs, W, cap, err);
+}
+
 __global__ __launch_bounds__(256) void crt_recon_carry_kernel(
     const uint32_t *__restrict__ Wv, const int32_t *__restrict__ Cb, int64_t ystride, int nw, int nrb,
     const int64_t *__restrict__ G, int gshift,
@@ -627,6 +640,19 @@ int reconstruct_with(const mgx_crt_plan *pl, int64_t K, int64_t N, const uint32_
 {
     const int64_t nent = K * N, ystride = crt_ystride(nent);
     const int nrb = recon_row_blocks(pl);
+    // one pass, no scratch, when the entry blocks alone (one workgroup each, every row block in turn) fill
+    // the device twice over; smaller products keep the row blocks' parallelism of the two passes
+    const int64_t nblk = ceil_div(nent, RECON_ENT);
+    const int64_t rmode = param(MGX_PARAM_RECON);
+    if (pl->S == 2 && 4 * pl->nw <= pl->Lbp && (rmode == 2 || (rmode == 0 && nblk >= 2 * device_cus()))) {
+        if (round_up(nblk, 8) > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "product too large for one reconstruct launch");
+        if (nblk > 0)
+            hipLaunchKernelGGL(crt_recon_fused_kernel, dim3((unsigned)round_up(nblk, 8)), dim3(256), 0, s, pl->d_pc,
+                               (int)pl->P, pl->d_mc, (int)pl->Kr, nrb, (int)pl->nw, Y, ystride, nent, pl->d_m32, N,
+                               (uint64_t *)C, Crs, cap + 1, cap, device_error_word(), nblk);
+        MGX_HIP(hipGetLastError());
+        return MGX_OK;
+    }
     uint32_t *Wv = (uint32_t *)scr;
     int32_t *Cb = (int32_t *)(scr + pl->nw * ystride * 4);
     int64_t *G = (int64_t *)(scr + (pl->nw + nrb) * ystride * 4);

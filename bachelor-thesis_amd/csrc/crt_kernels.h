@@ -1341,38 +1341,44 @@ static_assert(RECON_ROWS / 4 == 32, "the carry pass takes one row block of words
 // (S = 2: floor(2^45/p) for every digit row q < Kr/S, zero past the last prime)
 inline int64_t recon_res_lds(int64_t Kr, int64_t S) { return (int64_t)RECON_ROWS * (Kr + 16) + 4 * (Kr / S); }
 
-// Epilogue of a reconstruct tile (128 result bytes x 256 entries, lane
-// c32 + 32 h of wave w holds rows 8g + 4h + u of row tile rt for entries
-// e0 + 64 w + 32 ct + c32): the row block's 32 words of each entry, carried,
-// and its partial gamma sums.
+// The 32 words of a reconstruct tile's row block for the lane's entry
+// (128 result bytes x 256 entries; before the exchange, lane c32 + 32 h of
+// wave w holds rows 8g + 4h + u of row tile rt for entries e0 + 64 w + 32 ct
+// + c32).  Row 8g + 4h + u of row tile rt is byte u of word j = rt*8 + 2g + h
+// of the block.  Lane c32 + 32h ends up with all 32 words of entry e0 + 64
+// wave + 32 h + c32 (|word| < 2^49): v_permlane32_swap exchanges the upper
+// half-wave of the ct = 0 values with the lower half-wave of the ct = 1
+// values, after which the first operand holds word 2g' and the second word
+// 2g' + 1 of the lane's own entry.
+__device__ __forceinline__ void recon_words(const v16i (&acc)[4][2], int64_t (&wv)[32])
+{
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+            uint64_t v[2];
+#pragma unroll
+            for (int ct = 0; ct < 2; ct++)
+                v[ct] = (uint64_t)((int64_t)acc[rt][ct][4 * g] + ((int64_t)acc[rt][ct][4 * g + 1] << 8) +
+                                   ((int64_t)acc[rt][ct][4 * g + 2] << 16) +
+                                   ((int64_t)acc[rt][ct][4 * g + 3] << 24));
+            const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)v[0], (unsigned)v[1], false, false);
+            const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(v[0] >> 32), (unsigned)(v[1] >> 32),
+                                                             false, false);
+            wv[rt * 8 + 2 * g] = (int64_t)(((uint64_t)hi[0] << 32) | lo[0]);
+            wv[rt * 8 + 2 * g + 1] = (int64_t)(((uint64_t)hi[1] << 32) | lo[1]);
+        }
+}
+
+// Epilogue of a reconstruct tile: the row block's 32 words of each entry,
+// carried, and its partial gamma sums.
 __device__ __forceinline__ void recon_store(const v16i (&acc)[4][2], const int64_t (&gsum)[2], int64_t e0, int wave,
     int h, int c32, int rb, int nw, int nrb, int64_t nent, uint32_t *__restrict__ Wv, int32_t *__restrict__ Cb,
     int64_t *__restrict__ G)
 {
-    // Row 8g + 4h + u of row tile rt is byte u of word j = rt*8 + 2g + h of
-    // the block (word rb*32 + j).  Lane c32 + 32h ends up with all 32 words
-    // of entry e0 + 64 wave + 32 h + c32: v_permlane32_swap exchanges the
-    // upper half-wave of the ct = 0 values with the lower half-wave of the
-    // ct = 1 values, after which the first operand holds word 2g' and the
-    // second word 2g' + 1 of the lane's own entry.
     {
         int64_t wv[32];
-#pragma unroll
-        for (int rt = 0; rt < 4; rt++)
-#pragma unroll
-            for (int g = 0; g < 4; g++) {
-                uint64_t v[2];
-#pragma unroll
-                for (int ct = 0; ct < 2; ct++)
-                    v[ct] = (uint64_t)((int64_t)acc[rt][ct][4 * g] + ((int64_t)acc[rt][ct][4 * g + 1] << 8) +
-                                       ((int64_t)acc[rt][ct][4 * g + 2] << 16) +
-                                       ((int64_t)acc[rt][ct][4 * g + 3] << 24));
-                const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)v[0], (unsigned)v[1], false, false);
-                const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(v[0] >> 32), (unsigned)(v[1] >> 32),
-                                                                 false, false);
-                wv[rt * 8 + 2 * g] = (int64_t)(((uint64_t)hi[0] << 32) | lo[0]);
-                wv[rt * 8 + 2 * g + 1] = (int64_t)(((uint64_t)hi[1] << 32) | lo[1]);
-            }
+        recon_words(acc, wv);
         const int64_t e = e0 + 64 * wave + 32 * h + c32;
         const int nv = min(RECON_ROWS / 4, nw - rb * (RECON_ROWS / 4));   // words of this block below nw
         int64_t carry = 0;
@@ -1440,23 +1446,23 @@ __device__ __forceinline__ int64_t gamma_term(uint32_t wd, int sub, int q, int P
 constexpr int RS_STAGES = 3;
 constexpr int RS_YPITCH = 1056;
 constexpr int RS_STAGE = 128 * 64 + 16 * RS_YPITCH;
-__device__ __forceinline__ void recon_stream_tile(int64_t blk, const PrimeConst *__restrict__ pc, int P,
-    const int8_t *__restrict__ Mc, int Kr, int nrb, int nw, const uint32_t *__restrict__ Y, int64_t ystride,
-    int64_t nent, uint32_t *__restrict__ Wv, int32_t *__restrict__ Cb, int64_t *__restrict__ G)
+// One K sweep of a streaming tile: acc (128 result bytes from Mc row mrow0 on
+// x the 256 entries from e0 on) and the gamma terms of the steps gmode picks
+// (0: none, 1: k = rb mod nrb, 2: every step) added to gsum.  LS: the ring.
+__device__ __forceinline__ void recon_sweep(int8_t *LS, const PrimeConst *__restrict__ pc, int P,
+    const int8_t *__restrict__ Mc, int Kr, int64_t mrow0, const uint32_t *__restrict__ Y, int64_t ystride, int64_t e0,
+    int gmode, int rb, int nrb, v16i (&acc)[4][2], int64_t (&gsum)[2])
 {
     constexpr int PER = 6;
-    __shared__ __attribute__((aligned(1024))) int8_t LS[RS_STAGES * RS_STAGE];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, h = lane >> 5, c32 = lane & 31;
     const int wave_u = __builtin_amdgcn_readfirstlane(wave);
-    const int rb = (int)(blk % nrb);
-    const int64_t e0 = (blk / nrb) * RECON_ENT;
     const int nsteps = Kr / 64, nyp = (2 * P + 3) / 4;
     const uint32_t ls0 = __builtin_amdgcn_readfirstlane(lds_offset(LS));
     // Mc: lane l takes row 16 (2 wave + u) + l / 4, slot l % 4, i.e. chunk (l % 4) ^ ((l >> 4) & 3)
     const uint32_t mvoff = (uint32_t)((lane >> 2) * Kr + 16 * ((lane & 3) ^ ((lane >> 4) & 3)));
     const int8_t *mbase[2];
 #pragma unroll
-    for (int u = 0; u < 2; u++) mbase[u] = uniform_ptr(Mc + (int64_t)(rb * RECON_ROWS + 16 * (2 * wave_u + u)) * Kr);
+    for (int u = 0; u < 2; u++) mbase[u] = uniform_ptr(Mc + (mrow0 + 16 * (2 * wave_u + u)) * Kr);
     const int8_t *ybase = uniform_ptr((const int8_t *)(Y + e0));
 #define ISSUE(k_)                                                                                      \
     {                                                                                                  \
@@ -1470,12 +1476,10 @@ __device__ __forceinline__ void recon_stream_tile(int64_t blk, const PrimeConst 
         }                                                                                              \
     }
 
-    v16i acc[4][2];
 #pragma unroll
     for (int rt = 0; rt < 4; rt++)
 #pragma unroll
         for (int ct = 0; ct < 2; ct++) acc[rt][ct] = (v16i){0};
-    int64_t gsum[2] = {0, 0};
 #pragma unroll
     for (int k = 0; k < RS_STAGES - 1; k++)
         if (k < nsteps) { ISSUE(k); }
@@ -1499,7 +1503,7 @@ __device__ __forceinline__ void recon_stream_tile(int64_t blk, const PrimeConst 
                 af[kk][rt] = *(const v4i *)(sa + row * 64 + 16 * ((2 * kk + h) ^ g2_swz(row)));
             }
         }
-        if (k % nrb == rb) {   // this row block's share of the gamma sums
+        if (gmode == 2 || (gmode == 1 && k % nrb == rb)) {   // gamma terms of this step
 #pragma unroll
             for (int kk = 0; kk < 2; kk++)
 #pragma unroll
@@ -1523,6 +1527,19 @@ __device__ __forceinline__ void recon_stream_tile(int64_t blk, const PrimeConst 
                 for (int ct = 0; ct < 2; ct++) acc[rt][ct] = mfma32(af[kk][rt], yf[ct][kk], acc[rt][ct]);
     }
 #undef ISSUE
+}
+
+__device__ __forceinline__ void recon_stream_tile(int64_t blk, const PrimeConst *__restrict__ pc, int P,
+    const int8_t *__restrict__ Mc, int Kr, int nrb, int nw, const uint32_t *__restrict__ Y, int64_t ystride,
+    int64_t nent, uint32_t *__restrict__ Wv, int32_t *__restrict__ Cb, int64_t *__restrict__ G)
+{
+    __shared__ __attribute__((aligned(1024))) int8_t LS[RS_STAGES * RS_STAGE];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, h = lane >> 5, c32 = lane & 31;
+    const int rb = (int)(blk % nrb);
+    const int64_t e0 = (blk / nrb) * RECON_ENT;
+    v16i acc[4][2];
+    int64_t gsum[2] = {0, 0};
+    recon_sweep(LS, pc, P, Mc, Kr, (int64_t)rb * RECON_ROWS, Y, ystride, e0, 1, rb, nrb, acc, gsum);
     recon_store(acc, gsum, e0, wave, h, c32, rb, nw, nrb, nent, Wv, Cb, G);
 }
 
@@ -1769,6 +1786,150 @@ __device__ __forceinline__ void recon_carry_block(int64_t blk, const uint32_t *_
             }
         __syncthreads();
     }
+    if (valid) {
+        if (hi_nz) atomicOr(err, DEV_ERR_CAPACITY);
+        C[s_base[tid] + W - 1] = (uint64_t)(neg ? -(top + 1) : top + 1);
+    }
+}
+
+// ------------------------------------------------------------------------
+// 3c. one-pass reconstruct (S = 2).  A workgroup takes 256 entries (e0 ..)
+//     through every row block with recon_sweep and writes their limbs of
+//     C = X' - gamma M itself: no word buffer between two passes.  The row
+//     blocks are aligned to the top word: block nrb - 1 holds words nw - 32 ..
+//     nw - 1, block b >= 1 words nw - 32 (nrb - b) .., block 0 the rest from
+//     word 0 (nw - 32 (nrb - 1) of them; nw is even, so every block starts
+//     on a limb).  The top block is swept first, with every K step's gamma
+//     terms: it gives gamma and, from its words down, the sign of C as the
+//     carry pass finds it (once the top part H reaches 2^28 in magnitude the
+//     words below, |word| < 2^50, cannot change its sign); its words stay in
+//     registers, carried within the block.  While an entry's sign is still
+//     open (|C| below 2^(32 (nw - 32) + 28): products far below their
+//     bound, zero ones) the workgroup sweeps further blocks downwards, which
+//     only extend H (exact: raw words, no carries).  Then the blocks below
+//     the top are swept from block 0 up and turned into limbs of |C| with one
+//     running carry, each block's 16 limbs per entry leaving through LDS as
+//     128-byte runs, and the top block's words close the chain; its carry-out
+//     lies beyond the limbs (|C| < 2^(32 nw)), as in the carry pass.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ int recon_block_start(int b, int nw, int nrb) { return b == 0 ? 0 : nw - 32 * (nrb - b); }
+
+__device__ __forceinline__ void recon_fused_block(int64_t eb, const PrimeConst *__restrict__ pc, int P,
+    const int8_t *__restrict__ Mc, int Kr, int nrb, int nw, const uint32_t *__restrict__ Y, int64_t ystride,
+    int64_t nent, const uint32_t *__restrict__ M32, int64_t colsC, uint64_t *__restrict__ C, int64_t Crs, int64_t W,
+    int64_t cap, unsigned *__restrict__ err)
+{
+    __shared__ __attribute__((aligned(1024))) int8_t LS[RS_STAGES * RS_STAGE];   // the K ring, then limb staging
+    __shared__ int64_t s_base[RECON_ENT];
+    static_assert(RS_STAGES * RS_STAGE >= RECON_ENT * CARRY_PITCH * 8, "the limb staging reuses the ring");
+    const int tid = threadIdx.x, h = (tid & 63) >> 5;
+    const int64_t e0 = eb * RECON_ENT;
+    const int64_t e = e0 + tid;   // after recon_words, thread 64 wave + 32 h + c32 holds entry e0 + tid
+    const bool valid = e < nent;
+    s_base[tid] = valid ? ((e / colsC) * Crs + e % colsC) * W : -1;   // first read after the sweeps' barriers
+    const int T = nrb - 1;
+    v16i acc[4][2];
+    int64_t gsum[2] = {0, 0};
+    int64_t wv[32];
+
+    // the top block, with every step's gamma terms
+    const int wst = recon_block_start(T, nw, nrb), nvt = nw - wst;
+    recon_sweep(LS, pc, P, Mc, Kr, 4 * (int64_t)wst, Y, ystride, e0, 2, 0, nrb, acc, gsum);
+    recon_words(acc, wv);
+    int64_t gam;
+    {
+        // lanes l and l + 32 hold the two K halves of the same entries' sums
+        int64_t F[2];
+#pragma unroll
+        for (int ct = 0; ct < 2; ct++) {
+            const uint64_t b = (uint64_t)gsum[ct];
+            const uint32_t lo = __shfl_xor((unsigned)b, 32), hi = __shfl_xor((unsigned)(b >> 32), 32);
+            F[ct] = gsum[ct] + (int64_t)(((uint64_t)hi << 32) | lo);
+        }
+        gam = ((h ? F[1] : F[0]) + (1ll << 44)) >> 45;
+    }
+    uint32_t topw[32];   // the top block's words, carried within it
+    {
+        int64_t c = 0;
+#pragma unroll
+        for (int j = 0; j < 32; j++) {
+            const int64_t x = (j < nvt ? wv[j] : 0) + c;
+            c = x >> 32;
+            topw[j] = (uint32_t)x;
+        }
+    }
+    int64_t H = 0;
+    bool det = !valid;
+#pragma unroll
+    for (int j = 31; j >= 0; j--)
+        if (j < nvt && !det) {
+            H = H * 4294967296ll + (wv[j] - gam * (int64_t)M32[wst + j]);
+            det = H >= (1ll << 28) || H <= -(1ll << 28);
+        }
+    for (int b = T - 1; b >= 0 && __syncthreads_or(!det); b--) {
+        const int ws = recon_block_start(b, nw, nrb), nv = recon_block_start(b + 1, nw, nrb) - ws;
+        recon_sweep(LS, pc, P, Mc, Kr, 4 * (int64_t)ws, Y, ystride, e0, 0, 0, nrb, acc, gsum);
+        recon_words(acc, wv);
+#pragma unroll
+        for (int j = 31; j >= 0; j--)
+            if (j < nv && !det) {
+                H = H * 4294967296ll + (wv[j] - gam * (int64_t)M32[ws + j]);
+                det = H >= (1ll << 28) || H <= -(1ll << 28);
+            }
+    }
+    const bool neg = H < 0;   // H is C itself when no word fixed the sign
+    __syncthreads();   // the last sweep's ring reads are done before the next sweep's DMA
+
+    // limbs of |C|, block 0 up
+    const int nlimb = nw / 2, lim = (int)min<int64_t>(cap, nlimb);
+    const int64_t sgn = neg ? -1 : 1;
+    uint64_t *const LST = (uint64_t *)LS;
+    int64_t carry = 0, top = -1;
+    bool hi_nz = false;
+    uint64_t lb[16];
+    // one block's limbs (limbs L0 .. L0 + nl - 1) through LDS to C
+    auto emit = [&](int L0, int nl) {
+        __syncthreads();   // every wave is done with the ring's last stage
+#pragma unroll
+        for (int j = 0; j < 16; j++) LST[tid * CARRY_PITCH + j] = lb[j];
+        __syncthreads();
+        const int n = max(0, min(nl, lim - L0));
+        for (int f = tid; f < RECON_ENT * 16; f += RECON_ENT) {
+            const int ent = f >> 4, j = f & 15;
+            const int64_t base = s_base[ent];
+            if (j < n && base >= 0) C[base + L0 + j] = LST[ent * CARRY_PITCH + j];
+        }
+        __syncthreads();   // the staging is read before the next sweep refills the ring
+    };
+    // limbs of block (ws, nv) from its raw words, or from the top block's carried ones
+#define LIMBS(wsrc, ws, nv)                                                                            \
+    _Pragma("unroll") for (int j = 0; j < 16; j++) {                                                    \
+        uint64_t limb = 0;                                                                              \
+        if (2 * j < (nv)) {                                                                             \
+            const int64_t x0 = sgn * ((int64_t)wsrc[2 * j] - gam * (int64_t)M32[(ws) + 2 * j]) + carry; \
+            carry = x0 >> 32;                                                                           \
+            const int64_t x1 = sgn * ((int64_t)wsrc[2 * j + 1] - gam * (int64_t)M32[(ws) + 2 * j + 1]) + carry; \
+            carry = x1 >> 32;                                                                           \
+            limb = (uint64_t)(uint32_t)x0 | ((uint64_t)(uint32_t)x1 << 32);                             \
+            const int L = (ws) / 2 + j;                                                                 \
+            if (L < lim) {                                                                              \
+                if (limb) top = L;                                                                      \
+            } else if (L < nlimb) {                                                                     \
+                hi_nz |= limb != 0;                                                                     \
+            }                                                                                           \
+        }                                                                                               \
+        lb[j] = limb;                                                                                   \
+    }
+    for (int b = 0; b < T; b++) {
+        const int ws = recon_block_start(b, nw, nrb), nv = recon_block_start(b + 1, nw, nrb) - ws;
+        recon_sweep(LS, pc, P, Mc, Kr, 4 * (int64_t)ws, Y, ystride, e0, 0, 0, nrb, acc, gsum);
+        recon_words(acc, wv);
+        LIMBS(wv, ws, nv);
+        emit(ws / 2, nv / 2);
+    }
+    LIMBS(topw, wst, nvt);
+    emit(wst / 2, nvt / 2);
+#undef LIMBS
     if (valid) {
         if (hi_nz) atomicOr(err, DEV_ERR_CAPACITY);
         C[s_base[tid] + W - 1] = (uint64_t)(neg ? -(top + 1) : top + 1);

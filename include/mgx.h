@@ -292,10 +292,14 @@ typedef enum {
                                      product (default) */
     MGX_PARAM_GEMM_ROWS = 5,      /* residue-GEMM tile height of two-digit plans: 0 or 128 (default), or 64
                                      (twice the workgroups, for small products) */
-    MGX_PARAM_REDUCE = 6          /* reduce kernels of two-digit plans: 0 (default) = entries of at most
+    MGX_PARAM_REDUCE = 6,         /* reduce kernels of two-digit plans: 0 (default) = entries of at most
                                      256 bytes with the whole constant table in LDS (one workgroup per CU)
                                      whenever it fits, entries over 512 bytes as a tiled MFMA GEMM;
                                      1 = entry-resident tiles of 128 entries, and 64-row tiles past 512 bytes */
+    MGX_PARAM_RECON = 7           /* reconstruct of two-digit plans in single-product calls: 0 (default) = one
+                                     pass writing C's limbs when the product has at least 512 blocks of 256
+                                     entries per 256 CUs, else two; 1 = always a word-buffer GEMM pass and a
+                                     carry pass; 2 = always one pass */
 } mgx_param;
 int mgx_set_param(int param, int64_t value);
 int64_t mgx_get_param(int param);
