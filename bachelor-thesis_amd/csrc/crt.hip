@@ -640,11 +640,14 @@ int reconstruct_with(const mgx_crt_plan *pl, int64_t K, int64_t N, const uint32_
 {
     const int64_t nent = K * N, ystride = crt_ystride(nent);
     const int nrb = recon_row_blocks(pl);
-    // one pass, no scratch, when the entry blocks alone (one workgroup each, every row block in turn) fill
-    // the device twice over; smaller products keep the row blocks' parallelism of the two passes
+    // one pass, no scratch, when the entry blocks alone (one workgroup each, every row block in turn) give
+    // every CU one and the row blocks are few (measured: ahead from 256 blocks of 256 entries at 5 and 9 row
+    // blocks, behind at 17 and more, where a workgroup's sweeps run long and the last ones run alone);
+    // otherwise the two passes, which spread the row blocks over workgroups
     const int64_t nblk = ceil_div(nent, RECON_ENT);
     const int64_t rmode = param(MGX_PARAM_RECON);
-    if (pl->S == 2 && 4 * pl->nw <= pl->Lbp && (rmode == 2 || (rmode == 0 && nblk >= 2 * device_cus()))) {
+    if (pl->S == 2 && 4 * pl->nw <= pl->Lbp &&
+        (rmode == 2 || (rmode == 0 && nrb <= 9 && nblk >= device_cus()))) {
         if (round_up(nblk, 8) > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "product too large for one reconstruct launch");
         if (nblk > 0)
             hipLaunchKernelGGL(crt_recon_fused_kernel, dim3((unsigned)round_up(nblk, 8)), dim3(256), 0, s, pl->d_pc,

@@ -1,9 +1,10 @@
-"""The one-pass reconstruct (MGX_PARAM_RECON = 2 forces it; the default 0 takes it
-for two-digit products of at least two workgroups per CU) against the two-pass
-one (word buffer and carry pass, MGX_PARAM_RECON = 1) and against Python integers.  Cases: products near their bound (the top row block
-fixes every sign), products far below an explicit plan's bound and zero ones
-(the workgroup sweeps further blocks for the sign), row-block counts from 1 to
-over 30, entry counts that leave a partial block of 256, and the capacity check."""
+"""The one-pass reconstruct (MGX_PARAM_RECON = 2 forces it; the default 0 takes
+it for two-digit products of a workgroup per CU and at most 9 row blocks) against
+the two-pass one (word buffer and carry pass, MGX_PARAM_RECON = 1) and against
+Python integers.  Cases: products near their bound (the top row block fixes every
+sign), products far below an explicit plan's bound and zero ones (the workgroup
+sweeps further blocks for the sign), row-block counts from 1 to over 30, entry
+counts that leave a partial block of 256, and the capacity check."""
 import pytest
 
 from helpers import is_normalized, matmul, rand_ints, to_entries, to_ints
