@@ -182,6 +182,20 @@ __global__ __launch_bounds__(256, 2) void batch_recon_gemm_kernel(Queue q, const
     }
 }
 
+// two-digit products of at most 9 row blocks: the one-pass reconstruct, one item per 256 entries
+__global__ __launch_bounds__(256, 2) void batch_recon_fused_kernel(Queue q, const ReconJob *jobs, unsigned *err)
+{
+    int j;
+    int64_t t;
+    Claim cl;
+    while (next_item(q, cl, j, t)) {
+        const ReconJob &r = jobs[j];
+        recon_fused_block(t, r.pc, r.P, r.Mc, r.Kr, r.nrb, r.nw, r.Y, r.ystride, r.nent, r.M32, r.colsC, r.C, r.Crs,
+                          r.W, r.cap, err);
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(256) void batch_recon_carry_kernel(Queue q, const ReconJob *jobs, unsigned *err)
 {
     int j;
@@ -307,14 +321,15 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
     // kind 0: entries up to 256 bytes, 1: generic, 2: up to 512 bytes),
     // reconstruct-GEMM by slice count, residue GEMM by slice count and
     // chunking (2 sc + chunked); the carry pass takes every CRT product in one
-    // queue.
+    // queue; two-digit products of at most 9 row blocks take the one-pass
+    // reconstruct instead of the reconstruct GEMM and the carry pass.
     std::vector<ReduceJob> rj[6];
     // two-digit generic reduce jobs (class 3) as tiled MFMA GEMMs (fixed at creation: the graph is captured here)
     const bool rgemm = param(MGX_PARAM_REDUCE) == 0;
     std::vector<GemmJob> gj[4];
-    std::vector<ReconJob> cj[2], kj;
+    std::vector<ReconJob> cj[2], kj, fj;
     std::vector<PlainJob> pj[NCLASS];
-    std::vector<int64_t> rp[6], gp[4], cp[2], kp{0}, pp[NCLASS];
+    std::vector<int64_t> rp[6], gp[4], cp[2], kp{0}, fp{0}, pp[NCLASS];
     for (auto &v : rp) v.push_back(0);
     for (auto &v : gp) v.push_back(0);
     for (auto &v : cp) v.push_back(0);
@@ -322,7 +337,9 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
     struct Dims { int64_t rpadA, rpadB, kpad, ystride; };
     std::vector<Dims> dims(count);
     std::vector<int> pclass(count, -1), crt(count, 0);
-    size_t nred[6] = {0, 0, 0, 0, 0, 0}, ngem[4] = {0, 0, 0, 0}, nrec[2] = {0, 0}, nrec_all = 0;
+    size_t nred[6] = {0, 0, 0, 0, 0, 0}, ngem[4] = {0, 0, 0, 0}, nrec[2] = {0, 0}, nrec_all = 0, nfus = 0;
+    std::vector<int> fused(count, 0);
+    const bool fuse_ok = param(MGX_PARAM_RECON) != 1;
     size_t npl[NCLASS] = {0, 0, 0, 0, 0, 0};
     for (int64_t i = 0; i < count; i++) {
         const mgx_product &p = prods[i];
@@ -335,8 +352,14 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             crt[i] = 1;
             nred[2 * reduce_kind(views[i].kb) + sc] += 2;
             ngem[2 * sc + (d.kpad > (sc ? KCHUNK2 : KCHUNK))]++;
-            nrec[sc]++;
-            nrec_all++;
+            const PlanView &v = views[i];
+            fused[i] = fuse_ok && sc && ceil_div(4 * v.nw, RECON_ROWS) <= 9 && 4 * v.nw <= v.Lbp;
+            if (fused[i]) {
+                nfus++;
+            } else {
+                nrec[sc]++;
+                nrec_all++;
+            }
         } else {
             int64_t words = 2 * ((std::max(bitsA[i], bitsB[i]) + 63) / 64);
             int k = 0;
@@ -354,8 +377,10 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
         o_ra[i] = L.take(v.S * v.P * d.rpadA * d.kpad);
         o_rb[i] = L.take(v.S * v.P * d.rpadB * d.kpad);
         o_y[i] = L.take(ceil_div(v.S * v.P, 4) * d.ystride * 4);
-        o_wv[i] = L.take(v.nw * d.ystride * 4);
-        o_ga[i] = L.take(ceil_div(4 * v.nw, RECON_ROWS) * d.ystride * 12);   // G (int64), then Cb (int32)
+        if (!fused[i]) {   // the two passes' words, then G (int64) and Cb (int32)
+            o_wv[i] = L.take(v.nw * d.ystride * 4);
+            o_ga[i] = L.take(ceil_div(4 * v.nw, RECON_ROWS) * d.ystride * 12);
+        }
     }
     const int64_t o_ctr = L.take(NSLOTS * QRANGES * QPITCH * 8);
     int64_t o_rj[6], o_rp[6], o_gj[4], o_gp[4], o_cj[2], o_cp[2], o_pj[NCLASS], o_pp[NCLASS];
@@ -363,6 +388,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
     for (int c = 0; c < 4; c++) { o_gj[c] = L.take((int64_t)(ngem[c] * sizeof(GemmJob))); o_gp[c] = L.take((int64_t)(ngem[c] + 1) * 8); }
     for (int c = 0; c < 2; c++) { o_cj[c] = L.take((int64_t)(nrec[c] * sizeof(ReconJob))); o_cp[c] = L.take((int64_t)(nrec[c] + 1) * 8); }
     const int64_t o_kj = L.take((int64_t)(nrec_all * sizeof(ReconJob))), o_kp = L.take((int64_t)(nrec_all + 1) * 8);
+    const int64_t o_fj = L.take((int64_t)(nfus * sizeof(ReconJob))), o_fp = L.take((int64_t)(nfus + 1) * 8);
     for (int c = 0; c < NCLASS; c++) { o_pj[c] = L.take((int64_t)(npl[c] * sizeof(PlainJob))); o_pp[c] = L.take((int64_t)(npl[c] + 1) * 8); }
     b->scratch_bytes = L.off;
     if (hipMalloc(&b->scratch, (size_t)b->scratch_bytes) != hipSuccess) {
@@ -409,13 +435,19 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             c.pc = v.pc; c.Mc = v.mc; c.M32 = v.m32; c.Y = g.Y; c.P = (int)v.P; c.Kr = (int)v.Kr;
             c.nrb = (int)ceil_div(4 * v.nw, RECON_ROWS); c.nw = (int)v.nw; c.ystride = d.ystride;
             c.nent = p.K * p.N; c.colsC = p.N; c.Crs = p.Crs; c.W = p.cap + 1; c.cap = p.cap; c.C = (uint64_t *)p.C;
-            c.Wv = (uint32_t *)(base + o_wv[i]); c.G = (int64_t *)(base + o_ga[i]);
             c.gshift = v.S == 2 ? 45 : 32;
-            c.Cb = (int32_t *)(base + o_ga[i] + c.nrb * d.ystride * 8);
-            cj[sc].push_back(c);
-            cp[sc].push_back(cp[sc].back() + ceil_div(c.nent, RECON_ENT) * c.nrb);
-            kj.push_back(c);
-            kp.push_back(kp.back() + ceil_div(c.nent, CARRY_ENT));
+            if (fused[i]) {
+                c.Wv = nullptr; c.G = nullptr; c.Cb = nullptr;
+                fj.push_back(c);
+                fp.push_back(fp.back() + ceil_div(c.nent, RECON_ENT));
+            } else {
+                c.Wv = (uint32_t *)(base + o_wv[i]); c.G = (int64_t *)(base + o_ga[i]);
+                c.Cb = (int32_t *)(base + o_ga[i] + c.nrb * d.ystride * 8);
+                cj[sc].push_back(c);
+                cp[sc].push_back(cp[sc].back() + ceil_div(c.nent, RECON_ENT) * c.nrb);
+                kj.push_back(c);
+                kp.push_back(kp.back() + ceil_div(c.nent, CARRY_ENT));
+            }
         } else if (pclass[i] >= 0) {
             PlainJob q;
             q.K = p.K; q.M = p.M; q.N = p.N; q.Crs = p.Crs; q.Ars = p.Ars; q.Brs = p.Brs; q.W = p.cap + 1; q.cap = p.cap;
@@ -443,6 +475,8 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
     }
     if (st == MGX_OK) st = up(o_kj, kj.data(), kj.size() * sizeof(ReconJob));
     if (st == MGX_OK) st = up(o_kp, kp.data(), kp.size() * 8);
+    if (st == MGX_OK) st = up(o_fj, fj.data(), fj.size() * sizeof(ReconJob));
+    if (st == MGX_OK) st = up(o_fp, fp.data(), fp.size() * 8);
     for (int c = 0; c < NCLASS && st == MGX_OK; c++) {
         st = up(o_pj[c], pj[c].data(), pj[c].size() * sizeof(PlainJob));
         if (st == MGX_OK) st = up(o_pp[c], pp[c].data(), pp[c].size() * 8);
@@ -451,7 +485,8 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
 
     // ---- capture: reset counters, plain classes, reduce, residue GEMM, reconstruct (GEMM, carry).
     // Counter slots: 0-1 entry-resident reduce (256 bytes), 2-5 residue GEMM, 6-7 reconstruct GEMM, 8 carry,
-    // 9-14 plain classes, 15-16 generic reduce, 17-18 entry-resident reduce (512 bytes).
+    // 9-14 plain classes, 15-16 generic reduce, 17-18 entry-resident reduce (512 bytes), 19 one-pass
+    // reconstruct.
     unsigned long long *ctr = (unsigned long long *)(base + o_ctr);
     b->counters = ctr;
     unsigned *err = device_error_word();
@@ -512,6 +547,12 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             b->queues.push_back({MGX_QUEUE_RECONSTRUCT, 6 + c, cp[c].back(), grid});
             hipLaunchKernelGGL(c ? batch_recon_gemm_kernel<2> : batch_recon_gemm_kernel<4>, dim3(grid), dim3(256), 0,
                                cs, queue(6 + c, o_cp[c], cj[c].size()), (const ReconJob *)(base + o_cj[c]));
+        }
+        if (!fj.empty()) {
+            int grid = (int)std::min<int64_t>(fp.back(), 512);
+            b->queues.push_back({MGX_QUEUE_RECONSTRUCT, 19, fp.back(), grid});
+            hipLaunchKernelGGL(batch_recon_fused_kernel, dim3(grid), dim3(256), 0, cs, queue(19, o_fp, fj.size()),
+                               (const ReconJob *)(base + o_fj), err);
         }
         if (!kj.empty()) {
             int grid = (int)std::min<int64_t>(kp.back(), 1024);
