@@ -50,6 +50,13 @@ struct PlainJob {
     const uint64_t *A, *B;
     int toom;   // Toom-22 cutoff of this product's schedule
 };
+// C = A +- B over an H x Wd block (one Winograd block addition)
+struct AorsJob {
+    int64_t Wd, H, Crs, Ars, Brs, W, cap;
+    uint64_t *C;
+    const uint64_t *A, *B;
+    int sub;
+};
 
 struct Queue {
     unsigned long long *counter;   // QRANGES counters, QPITCH apart
@@ -82,7 +89,7 @@ __device__ __forceinline__ bool next_item(const Queue &q, Claim &cl, int &job, i
     return it >= 0 && queue_map(q.prefix, q.njobs, it, &job, &tile);
 }
 
-constexpr int NSLOTS = 32;   // queues of a batch (counter slots)
+constexpr int NSLOTS = 40;   // queues of a batch (counter slots)
 __global__ void reset_counters_kernel(unsigned long long *c, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -221,6 +228,19 @@ __global__ __launch_bounds__(256) void batch_plain_kernel(Queue q, const PlainJo
     }
 }
 
+// tile = four entries, one per wavefront
+__global__ __launch_bounds__(256) void batch_aors_kernel(Queue q, const AorsJob *jobs, unsigned *err)
+{
+    int j;
+    int64_t t;
+    Claim cl;
+    while (next_item(q, cl, j, t)) {
+        const AorsJob &a = jobs[j];
+        aors_entry<false>(4 * t + (threadIdx.x >> 6), a.Wd, a.H, a.C, a.Crs, a.A, a.Ars, a.B, a.Brs, a.W, a.cap,
+                          a.sub, err);
+    }
+}
+
 }  // namespace mgx
 
 using namespace mgx;
@@ -263,6 +283,116 @@ void launch_plain(int grid, hipStream_t s, Queue q, const PlainJob *jobs, unsign
     hipLaunchKernelGGL(batch_plain_kernel<NB>, dim3(grid), dim3(256), 0, s, q, jobs, err);
 }
 
+// plain kernel class (blocks of 64 words) for operands of opbits bits
+int plain_class(int64_t opbits)
+{
+    const int64_t words = 2 * ((opbits + 63) / 64);
+    int k = 0;
+    while (k < NCLASS - 1 && 64 * NB_CLASSES[k] < words) k++;
+    return k;
+}
+
+// Winograd levels of a plain-routed product as batch jobs: the seven
+// products and the block additions of mat-gpu/mat-defines.h:144-164 (the
+// recursion rec.cpp runs one launch at a time) regrouped into stages whose
+// jobs are independent, so that each stage is one queue for the whole batch:
+//   pre-adds   S1 = A21 + A22, S3 = A11 - A21, T1 = B12 - B11, T3 = B22 - B12
+//              S2 = S1 - A11, T2 = B22 - T1
+//              S4 = A12 - S2, T4 = T2 - B21
+//   products   P1 = A11 B11, P2 = A12 B21, P3 = S4 B22, P4 = A22 T4,
+//              P5 = S1 T1, P6 = S2 T2, P7 = S3 T3
+//   post-adds  C11 = P1 + P2, U2 = P1 + P6 (into P6)
+//              U3 = U2 + P7 (into P7), U4 = U2 + P5 (into P1)
+//              C12 = U4 + P3, C21 = U3 - P4, C22 = U3 + P5
+// A level splits the even part of K x M x N; an odd last row of C and an odd
+// last column are plain products of their own, and an odd inner dimension
+// leaves the rank-one term A[:, M-1] B[M-1, :], a plain product into a
+// temporary added to C after the level's post-adds (the fix-up).  Stage
+// numbers: pre-adds 3L + {0, 1, 2} for level L = 0, 1; then, inner level
+// first, the post-adds and the fix-up at 6 + 4 (1 - L) + {0, 1, 2, 3}.  The
+// products of the last level run in the plain kernel queues between.
+constexpr int WSTAGES = 14;
+struct WinoGen {
+    int depth, toom;
+    int64_t W, cap, bits;   // entry words, limbs, operand bit bound at level 0
+    uintptr_t tmp;          // the product's temporaries (0 while sizing)
+    int64_t used = 0;
+    std::vector<AorsJob> *stage;                        // WSTAGES lists
+    std::vector<std::pair<PlainJob, int64_t>> leaves;   // products and their operand bits
+
+    uint64_t *take(int64_t entries)
+    {
+        uint64_t *p = (uint64_t *)(tmp + used);
+        used += round_up(std::max<int64_t>(entries, 1) * W * 8, 256);
+        return p;
+    }
+    template <class T>
+    T *at(T *X, int64_t rs, int64_t y, int64_t x) const { return X + (y * rs + x) * W; }
+    void mul(int64_t K, int64_t M, int64_t N, uint64_t *C, int64_t Crs, const uint64_t *A, int64_t Ars,
+             const uint64_t *B, int64_t Brs, int64_t opbits)
+    {
+        PlainJob q;
+        q.K = K; q.M = M; q.N = N; q.Crs = Crs; q.Ars = Ars; q.Brs = Brs; q.W = W; q.cap = cap;
+        q.C = C; q.A = A; q.B = B; q.toom = toom;
+        leaves.push_back({q, opbits});
+    }
+    void add(int st, int64_t H, int64_t Wd, uint64_t *C, int64_t Crs, const uint64_t *A, int64_t Ars,
+             const uint64_t *B, int64_t Brs, bool sub)
+    {
+        AorsJob a;
+        a.Wd = Wd; a.H = H; a.Crs = Crs; a.Ars = Ars; a.Brs = Brs; a.W = W; a.cap = cap;
+        a.C = C; a.A = A; a.B = B; a.sub = sub;
+        stage[st].push_back(a);
+    }
+    void gen(int level, int64_t K, int64_t M, int64_t N, uint64_t *C, int64_t Crs, const uint64_t *A, int64_t Ars,
+             const uint64_t *B, int64_t Brs)
+    {
+        const int64_t ob = bits + 2 * level;   // operand sums grow by two bits a level
+        if (level == depth || K < 2 || M < 2 || N < 2) {
+            mul(K, M, N, C, Crs, A, Ars, B, Brs, ob);
+            return;
+        }
+        const int64_t K2 = K & ~1ll, M2 = M & ~1ll, N2 = N & ~1ll, h = K2 / 2, m = M2 / 2, w = N2 / 2;
+        const int pre = 3 * level, post = 6 + 4 * (1 - level);
+        if (K & 1) mul(1, M, N, at(C, Crs, K2, 0), Crs, at(A, Ars, K2, 0), Ars, B, Brs, ob);
+        if (N & 1) mul(K2, M, 1, at(C, Crs, 0, N2), Crs, A, Ars, at(B, Brs, 0, N2), Brs, ob);
+        if (M & 1) {
+            uint64_t *R = take(K2 * N2);
+            mul(K2, 1, N2, R, N2, at(A, Ars, 0, M2), Ars, at(B, Brs, M2, 0), Brs, ob);
+            add(post + 3, K2, N2, C, Crs, C, Crs, R, N2, false);
+        }
+        const uint64_t *A11 = A, *A12 = at(A, Ars, 0, m), *A21 = at(A, Ars, h, 0), *A22 = at(A, Ars, h, m);
+        const uint64_t *B11 = B, *B12 = at(B, Brs, 0, w), *B21 = at(B, Brs, m, 0), *B22 = at(B, Brs, m, w);
+        uint64_t *S[4], *T[4], *P[7];
+        for (auto &x : S) x = take(h * m);
+        for (auto &x : T) x = take(m * w);
+        for (auto &x : P) x = take(h * w);
+        add(pre, h, m, S[0], m, A21, Ars, A22, Ars, false);
+        add(pre, h, m, S[2], m, A11, Ars, A21, Ars, true);
+        add(pre, m, w, T[0], w, B12, Brs, B11, Brs, true);
+        add(pre, m, w, T[2], w, B22, Brs, B12, Brs, true);
+        add(pre + 1, h, m, S[1], m, S[0], m, A11, Ars, true);
+        add(pre + 1, m, w, T[1], w, B22, Brs, T[0], w, true);
+        add(pre + 2, h, m, S[3], m, A12, Ars, S[1], m, true);
+        add(pre + 2, m, w, T[3], w, T[1], w, B21, Brs, true);
+        gen(level + 1, h, m, w, P[0], w, A11, Ars, B11, Brs);
+        gen(level + 1, h, m, w, P[1], w, A12, Ars, B21, Brs);
+        gen(level + 1, h, m, w, P[2], w, S[3], m, B22, Brs);
+        gen(level + 1, h, m, w, P[3], w, A22, Ars, T[3], w);
+        gen(level + 1, h, m, w, P[4], w, S[0], m, T[0], w);
+        gen(level + 1, h, m, w, P[5], w, S[1], m, T[1], w);
+        gen(level + 1, h, m, w, P[6], w, S[2], m, T[2], w);
+        uint64_t *C11 = C, *C12 = at(C, Crs, 0, w), *C21 = at(C, Crs, h, 0), *C22 = at(C, Crs, h, w);
+        add(post, h, w, C11, Crs, P[0], w, P[1], w, false);
+        add(post, h, w, P[5], w, P[0], w, P[5], w, false);
+        add(post + 1, h, w, P[6], w, P[5], w, P[6], w, false);
+        add(post + 1, h, w, P[0], w, P[5], w, P[4], w, false);
+        add(post + 2, h, w, C12, Crs, P[0], w, P[2], w, false);
+        add(post + 2, h, w, C21, Crs, P[6], w, P[3], w, true);
+        add(post + 2, h, w, C22, Crs, P[6], w, P[4], w, false);
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -276,7 +406,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
     b->routes.assign(count, MGX_ROUTE_CRT);
     int st = MGX_OK;
     std::vector<int64_t> bitsA(count), bitsB(count);
-    std::vector<int> slices(count, -1), tooms(count, (int)param(MGX_PARAM_TOOM22));
+    std::vector<int> slices(count, -1), tooms(count, (int)param(MGX_PARAM_TOOM22)), wdepth(count, 0);
     std::vector<PlanView> views(count);
     for (int64_t i = 0; i < count && st == MGX_OK; i++) {
         const mgx_product &p = prods[i];
@@ -300,8 +430,16 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             slices[i] = sc.slices;   // the batch's residue GEMM has one tile shape
             tooms[i] = sc.toom;
         }
-        // the batch runs plain products as one flat per-entry launch
-        if (r == MGX_ROUTE_WINOGRAD1 || r == MGX_ROUTE_WINOGRAD2) r = MGX_ROUTE_PLAIN;
+        if (r == MGX_ROUTE_WINOGRAD1 || r == MGX_ROUTE_WINOGRAD2) {
+            // fewer levels when the leaf operands (two bits more a level)
+            // outgrow the plain kernel or the temporaries (at most
+            // M 2^(bitsA + bitsB + 3 levels + 2)) outgrow the capacity
+            const int64_t mb = std::max(bitsA[i], bitsB[i]), lgM = 64 - __builtin_clzll((uint64_t)std::max<int64_t>(p.M, 1));
+            int d = r == MGX_ROUTE_WINOGRAD1 ? 1 : 2;
+            while (d > 0 && ((mb + 2 * d + 63) / 64 * 2 > 64 * 8 || 64 * p.cap < bitsA[i] + bitsB[i] + 3 * d + 3 + lgM)) d--;
+            r = d == 0 ? MGX_ROUTE_PLAIN : d == 1 ? MGX_ROUTE_WINOGRAD1 : MGX_ROUTE_WINOGRAD2;
+            wdepth[i] = d;
+        }
         if (r == MGX_ROUTE_PLAIN && (std::max(bitsA[i], bitsB[i]) + 63) / 64 * 2 > 64 * 8)
             r = MGX_ROUTE_CRT;   // beyond the plain kernel's widest class
         b->routes[i] = r;
@@ -341,6 +479,18 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
     std::vector<int> fused(count, 0);
     const bool fuse_ok = param(MGX_PARAM_RECON) != 1;
     size_t npl[NCLASS] = {0, 0, 0, 0, 0, 0};
+    std::vector<AorsJob> aj[WSTAGES];
+    std::vector<int64_t> wbytes(count, 0);
+    auto wino = [&](int64_t i, uintptr_t tmp) {
+        const mgx_product &p = prods[i];
+        WinoGen g;
+        g.depth = wdepth[i]; g.toom = tooms[i]; g.W = p.cap + 1; g.cap = p.cap;
+        g.bits = std::max(bitsA[i], bitsB[i]);
+        g.tmp = tmp;
+        g.stage = aj;
+        g.gen(0, p.K, p.M, p.N, (uint64_t *)p.C, p.Crs, (const uint64_t *)p.A, p.Ars, (const uint64_t *)p.B, p.Brs);
+        return g;
+    };
     for (int64_t i = 0; i < count; i++) {
         const mgx_product &p = prods[i];
         if (p.K == 0 || p.N == 0) continue;
@@ -360,13 +510,20 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
                 nrec[sc]++;
                 nrec_all++;
             }
+        } else if (wdepth[i] > 0) {
+            const WinoGen g = wino(i, 0);   // sizing: temporaries from offset 0, jobs counted
+            wbytes[i] = g.used;
+            for (auto &lf : g.leaves) npl[plain_class(lf.second)]++;
+            pclass[i] = 0;
         } else {
-            int64_t words = 2 * ((std::max(bitsA[i], bitsB[i]) + 63) / 64);
-            int k = 0;
-            while (k < NCLASS - 1 && 64 * NB_CLASSES[k] < words) k++;
-            pclass[i] = k;
-            npl[k]++;
+            pclass[i] = plain_class(std::max(bitsA[i], bitsB[i]));
+            npl[pclass[i]]++;
         }
+    }
+    size_t nas[WSTAGES];
+    for (int st2 = 0; st2 < WSTAGES; st2++) {
+        nas[st2] = aj[st2].size();
+        aj[st2].clear();
     }
     Layout L;
     std::vector<int64_t> o_ra(count), o_rb(count), o_y(count), o_wv(count), o_ga(count);
@@ -382,7 +539,12 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             o_ga[i] = L.take(ceil_div(4 * v.nw, RECON_ROWS) * d.ystride * 12);
         }
     }
+    std::vector<int64_t> o_wt(count, 0);
+    for (int64_t i = 0; i < count; i++)
+        if (wbytes[i]) o_wt[i] = L.take(wbytes[i]);
     const int64_t o_ctr = L.take(NSLOTS * QRANGES * QPITCH * 8);
+    int64_t o_aj[WSTAGES], o_ap[WSTAGES];
+    for (int c = 0; c < WSTAGES; c++) { o_aj[c] = L.take((int64_t)(nas[c] * sizeof(AorsJob))); o_ap[c] = L.take((int64_t)(nas[c] + 1) * 8); }
     int64_t o_rj[6], o_rp[6], o_gj[4], o_gp[4], o_cj[2], o_cp[2], o_pj[NCLASS], o_pp[NCLASS];
     for (int c = 0; c < 6; c++) { o_rj[c] = L.take((int64_t)(nred[c] * sizeof(ReduceJob))); o_rp[c] = L.take((int64_t)(nred[c] + 1) * 8); }
     for (int c = 0; c < 4; c++) { o_gj[c] = L.take((int64_t)(ngem[c] * sizeof(GemmJob))); o_gp[c] = L.take((int64_t)(ngem[c] + 1) * 8); }
@@ -448,6 +610,13 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
                 kj.push_back(c);
                 kp.push_back(kp.back() + ceil_div(c.nent, CARRY_ENT));
             }
+        } else if (pclass[i] >= 0 && wdepth[i] > 0) {
+            const WinoGen g = wino(i, (uintptr_t)(base + o_wt[i]));
+            for (auto &lf : g.leaves) {
+                const int k = plain_class(lf.second);
+                pj[k].push_back(lf.first);
+                pp[k].push_back(pp[k].back() + ceil_div(lf.first.K * lf.first.N, 4));
+            }
         } else if (pclass[i] >= 0) {
             PlainJob q;
             q.K = p.K; q.M = p.M; q.N = p.N; q.Crs = p.Crs; q.Ars = p.Ars; q.Brs = p.Brs; q.W = p.cap + 1; q.cap = p.cap;
@@ -456,6 +625,12 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             pj[pclass[i]].push_back(q);
             pp[pclass[i]].push_back(pp[pclass[i]].back() + ceil_div(p.K * p.N, 4));
         }
+    }
+    std::vector<int64_t> ap[WSTAGES];
+    for (int c = 0; c < WSTAGES; c++) {
+        if (aj[c].size() != nas[c]) st = fail(MGX_ERR_UNSUPPORTED, "batch: Winograd stage %d has %zu jobs, sized for %zu", c, aj[c].size(), nas[c]);
+        ap[c].push_back(0);
+        for (const AorsJob &a : aj[c]) ap[c].push_back(ap[c].back() + ceil_div(a.Wd * a.H, 4));
     }
     auto up = [&](int64_t o, const void *src, size_t n) -> int {
         if (n) MGX_HIP(hipMemcpy(base + o, src, n, hipMemcpyHostToDevice));
@@ -481,12 +656,16 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
         st = up(o_pj[c], pj[c].data(), pj[c].size() * sizeof(PlainJob));
         if (st == MGX_OK) st = up(o_pp[c], pp[c].data(), pp[c].size() * 8);
     }
+    for (int c = 0; c < WSTAGES && st == MGX_OK; c++) {
+        st = up(o_aj[c], aj[c].data(), aj[c].size() * sizeof(AorsJob));
+        if (st == MGX_OK) st = up(o_ap[c], ap[c].data(), ap[c].size() * 8);
+    }
     if (st != MGX_OK) { mgx_batch_destroy(b); return st; }
 
     // ---- capture: reset counters, plain classes, reduce, residue GEMM, reconstruct (GEMM, carry).
     // Counter slots: 0-1 entry-resident reduce (256 bytes), 2-5 residue GEMM, 6-7 reconstruct GEMM, 8 carry,
     // 9-14 plain classes, 15-16 generic reduce, 17-18 entry-resident reduce (512 bytes), 19 one-pass
-    // reconstruct.
+    // reconstruct, 20-33 the Winograd addition stages (pre-adds before the plain classes, post-adds after).
     unsigned long long *ctr = (unsigned long long *)(base + o_ctr);
     b->counters = ctr;
     unsigned *err = device_error_word();
@@ -503,6 +682,14 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
     if (e == hipSuccess) {
         hipLaunchKernelGGL(reset_counters_kernel, dim3(NSLOTS * QRANGES * QPITCH / 256), dim3(256), 0, cs, ctr,
                            NSLOTS * QRANGES * QPITCH);
+        auto adds = [&](int c) {
+            if (aj[c].empty()) return;
+            int grid = (int)std::min<int64_t>(ap[c].back(), 1024);
+            b->queues.push_back({MGX_QUEUE_ADDSUB, 20 + c, ap[c].back(), grid});
+            hipLaunchKernelGGL(batch_aors_kernel, dim3(grid), dim3(256), 0, cs, queue(20 + c, o_ap[c], aj[c].size()),
+                               (const AorsJob *)(base + o_aj[c]), err);
+        };
+        for (int c = 0; c < 6; c++) adds(c);
         for (int c = 0; c < NCLASS; c++) {
             if (pj[c].empty()) continue;
             Queue q = queue(9 + c, o_pp[c], pj[c].size());
@@ -518,6 +705,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             default: launch_plain<8>(grid, cs, q, jobs, err); break;
             }
         }
+        for (int c = 6; c < WSTAGES; c++) adds(c);
         for (int c = 0; c < 6; c++) {
             if (rj[c].empty()) continue;
             const int kind = c >> 1, two = c & 1;

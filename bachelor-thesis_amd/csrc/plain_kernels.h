@@ -373,4 +373,60 @@ __device__ __forceinline__ void plain_entries(int64_t blk, int64_t K, int64_t M,
     if (l == 0) put_size<STRIDED>(C, Crs, y, x, W, cmp >= 0 ? limbs : -limbs);
 }
 
+// Entry idx (row-major over an H x Wd block) of C = A + (sub ? -B : B), one
+// wavefront per entry, mpz_add semantics (mat-gpu/mpz-add.cu:48-123).
+template <bool STRIDED>
+__device__ __forceinline__ void aors_entry(int64_t idx, int64_t Wd, int64_t H, uint64_t *__restrict__ C, int64_t Crs,
+                                           const uint64_t *__restrict__ A, int64_t Ars,
+                                           const uint64_t *__restrict__ B, int64_t Brs, int64_t W, int64_t cap,
+                                           int sub, unsigned *__restrict__ err)
+{
+    if (idx >= Wd * H) return;
+    const int64_t y = idx / Wd, x = idx % Wd;
+    const uint32_t l = lane_id();
+    Ent<STRIDED> ea = ent<STRIDED>(A, Ars, y, x, W), eb = ent<STRIDED>(B, Brs, y, x, W);
+    int64_t sa = ea.size(), sb = eb.size();
+    if (sub) sb = -sb;
+    const int64_t na = 2 * (sa < 0 ? -sa : sa), nb = 2 * (sb < 0 ? -sb : sb);
+    const int ng = (int)((std::max(na, nb) + 64) >> 6);   // one extra word for the carry
+    const bool same = (sa >= 0) == (sb >= 0) || sa == 0 || sb == 0;
+    // pass 1 (different signs): compare magnitudes from the top group down
+    int cmp = 0;
+    if (!same)
+        for (int g = ng - 1; g >= 0 && !cmp; g--) {
+            int64_t w = 64 * g + l;
+            uint32_t u = w < na ? ea.word(w) : 0, v = w < nb ? eb.word(w) : 0;
+            uint64_t ne = ballot(u != v);
+            if (ne) {
+                int top = 63 - __clzll(ne);
+                cmp = rdlane(u, top) > rdlane(v, top) ? 1 : -1;
+            }
+        }
+    // pass 2: low to high; group g is written after it is read, and C may
+    // alias A or B only entry-for-entry, so later groups are still intact
+    int topw = -1;
+    uint32_t cy = 0, lastc = 0;
+    for (int g = 0; g < ng; g++) {
+        int64_t w = 64 * g + l;
+        uint32_t u = w < na ? ea.word(w) : 0, v = w < nb ? eb.word(w) : 0;
+        uint32_t r;
+        if (same) {
+            uint64_t s = (uint64_t)u + v;
+            uint32_t c = (uint32_t)(s >> 32);
+            uint32_t xin = shfl_up1(c, lastc);   // the raw carry of lane l enters lane l+1
+            lastc = rdlane(c, 63);
+            r = resolve_add((uint32_t)s, xin, cy);
+        } else {
+            r = cmp >= 0 ? resolve_sub(u, v, cy) : resolve_sub(v, u, cy);
+        }
+        uint64_t nz = ballot(r != 0);
+        if (nz) topw = 64 * g + 63 - __clzll(nz);
+        if (w < 2 * cap) put_word<STRIDED>(C, Crs, y, x, W, w, r);
+    }
+    const int64_t limbs = (topw + 2) / 2;
+    if (limbs > cap) { if (l == 0) atomicOr(err, DEV_ERR_CAPACITY); return; }
+    bool negr = same ? (sa < 0 || (sa == 0 && sb < 0)) : (cmp >= 0 ? sa < 0 : sb < 0);
+    if (l == 0) put_size<STRIDED>(C, Crs, y, x, W, negr ? -limbs : limbs);
+}
+
 }  // namespace mgx

@@ -454,7 +454,7 @@ def run_batch(args):
                                % (len(specs), args.chunk)},
         "products_per_s": len(specs) / total_s,
         "entry_products_per_s": entry_products / total_s,
-        "routes": {"crt": routes[0], "plain": routes[1] + routes[2] + routes[3]},
+        "routes": {"crt": routes[0], "plain": routes[1], "winograd1": routes[2], "winograd2": routes[3]},
         "scratch_bytes_max": scratch_max,
         "bit_exact": {"vs_single_product_crt": exact, "products_checked": checked},
     }

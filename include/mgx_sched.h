@@ -156,7 +156,8 @@ int mgx_batch_destroy(mgx_batch *b);
  * scheduler, mat-gpu/job.cu:183-336, has none). */
 typedef enum {
     MGX_QUEUE_PLAIN = 0, MGX_QUEUE_REDUCE = 1, MGX_QUEUE_RESIDUE_GEMM = 2, MGX_QUEUE_RECONSTRUCT = 3,
-    MGX_QUEUE_CARRY = 4
+    MGX_QUEUE_CARRY = 4,
+    MGX_QUEUE_ADDSUB = 5   /* block additions of the Winograd levels of plain products */
 } mgx_queue_kind;
 int64_t mgx_batch_queue_count(const mgx_batch *b);
 int mgx_batch_queue_info(const mgx_batch *b, int64_t i, int *kind, int64_t *items, int *grid, int64_t *claims);

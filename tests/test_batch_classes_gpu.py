@@ -8,8 +8,9 @@ Classes reached: entry-resident reduce (entries up to 256 bytes and up to
 512 bytes), generic reduce (entries over 512 bytes), residue GEMM with two
 and with four digits per residue, both with K beyond one int32 accumulation
 chunk, reconstruct for both digit counts, the carry pass, rectangular
-K x 2K x K products, empty products, and the plain per-entry kernel at
-several block classes (up to 8 blocks of 64 words).
+K x 2K x K products, empty products, the plain per-entry kernel at
+several block classes (up to 8 blocks of 64 words), and plain products with
+one or two Winograd levels (block-addition stages around the plain kernel).
 """
 import ctypes
 
@@ -31,8 +32,9 @@ def cap_for(bits, M):
 def build(mgx, specs, seed0):
     """Device buffers and mgx_product records for (K, M, N, bits, route) specs."""
     bufs, prods, wants = [], [], []
-    for i, (K, M, N, bits, route) in enumerate(specs):
-        cap = cap_for(bits, M)
+    for i, spec in enumerate(specs):
+        K, M, N, bits, route = spec[:5]
+        cap = spec[5] if len(spec) > 5 else cap_for(bits, M)
         A, B = rand_ints(K, M, bits, seed0 + 2 * i), rand_ints(M, N, bits, seed0 + 2 * i + 1)
         if K and M:
             A[0][0] = (1 << bits) - 1            # all-ones magnitude
@@ -123,6 +125,35 @@ def test_batch_every_class_four_digits(gpu):
         assert routes == [CRT] * 5
     finally:
         mgx.call("mgx_set_param", 3, old)
+
+
+def test_batch_winograd_levels(gpu):
+    """Plain products with one or two Winograd levels run as batch stages:
+    block additions in their own queues around the plain kernel's queues.
+    Odd K, M and N at either level (peeled rows and columns, the rank-one
+    fix-up), shapes too small to split, and the fall-backs: a leaf past the
+    plain kernel's widest class, and a capacity with no room for the
+    temporaries, run with fewer levels."""
+    mgx = gpu
+    L = mgx.bind_sched()
+    W1, W2 = 2, 3
+    specs = [
+        (16, 16, 16, 300, W1),
+        (17, 9, 13, 700, W1),        # odd K, M and N
+        (20, 12, 28, 2000, W2),      # even at level 0, odd at level 1
+        (9, 18, 9, 1500, W2),        # rectangular, odd outside
+        (12, 11, 10, 4000, W2),      # odd inner dimension at level 0
+        (2, 2, 2, 64, W2),           # level 1 is 1 x 1 x 1
+        (1, 5, 7, 100, W1),          # one row: no split
+        (6, 6, 6, 15000, W1),        # leaves of 8 blocks
+        (4, 6, 4, 16382, W2),        # second-level leaves would pass 512 words: one level
+        (5, 6, 4, 16383, W2),        # first-level leaves would too: flat plain kernel
+        (8, 8, 8, 1020, W1, 32),     # 2044-bit products in 32 limbs: no room for the temporaries
+        (24, 24, 24, 1500, CRT),     # a multi-modular product in the same batch
+    ]
+    bufs, prods, wants = build(mgx, specs, 900)
+    routes = run_and_check(mgx, L, bufs, prods, wants)
+    assert routes == [W1, W1, W2, W2, W2, W2, W1, W1, W1, PLAIN, PLAIN, CRT]
 
 
 def test_batch_plans_survive_cache_turnover(gpu):
