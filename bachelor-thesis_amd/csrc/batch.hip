@@ -43,6 +43,7 @@ struct ReconJob {
     int32_t *Cb;   // row-block carries, nrb x ystride
     int64_t *G;    // fixed-point gamma partial sums, nrb x ystride
     int gshift;    // their fraction bits
+    int gw;        // one-pass reconstruct: words of gamma rows above nw in Mc (0 or 2)
 };
 struct PlainJob {
     int64_t K, M, N, Crs, Ars, Brs, W, cap;
@@ -197,8 +198,8 @@ __global__ __launch_bounds__(256, 2) void batch_recon_fused_kernel(Queue q, cons
     Claim cl;
     while (next_item(q, cl, j, t)) {
         const ReconJob &r = jobs[j];
-        recon_fused_block(t, r.pc, r.P, r.Mc, r.Kr, r.nrb, r.nw, r.Y, r.ystride, r.nent, r.M32, r.colsC, r.C, r.Crs,
-                          r.W, r.cap, err);
+        recon_fused_block(t, r.pc, r.P, r.Mc, r.Kr, r.nrb, r.nw, r.gw, r.Y, r.ystride, r.nent, r.M32, r.colsC, r.C,
+                          r.Crs, r.W, r.cap, err);
         __syncthreads();
     }
 }
@@ -598,8 +599,10 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             c.nrb = (int)ceil_div(4 * v.nw, RECON_ROWS); c.nw = (int)v.nw; c.ystride = d.ystride;
             c.nent = p.K * p.N; c.colsC = p.N; c.Crs = p.Crs; c.W = p.cap + 1; c.cap = p.cap; c.C = (uint64_t *)p.C;
             c.gshift = v.S == 2 ? 45 : 32;
+            c.gw = 0;
             if (fused[i]) {
                 c.Wv = nullptr; c.G = nullptr; c.Cb = nullptr;
+                if (v.gw) { c.Mc = v.mcg; c.gw = (int)v.gw; }
                 fj.push_back(c);
                 fp.push_back(fp.back() + ceil_div(c.nent, RECON_ENT));
             } else {

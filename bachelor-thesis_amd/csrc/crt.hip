@@ -200,12 +200,12 @@ __global__ __launch_bounds__(256, 2) void crt_recon_gemm_kernel(
 __global__ __launch_bounds__(256, 2) void crt_recon_fused_kernel(
     const PrimeConst *__restrict__ pc, int P, const int8_t *__restrict__ Mc, int Kr, int nrb, int nw,
     const uint32_t *__restrict__ Y, int64_t ystride, int64_t nent, const uint32_t *__restrict__ M32, int64_t colsC,
-    uint64_t *__restrict__ C, int64_t Crs, int64_t W, int64_t cap, unsigned *__restrict__ err, int64_t nblk)
+    uint64_t *__restrict__ C, int64_t Crs, int64_t W, int64_t cap, unsigned *__restrict__ err, int64_t nblk, int gw)
 {
     const int64_t per = (nblk + 7) / 8;
     const int64_t v = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
     if (v >= nblk) return;
-    recon_fused_block(v, pc, P, Mc, Kr, nrb, nw, Y, ystride, nent, M32, colsC, C, Crs, W, cap, err);
+    recon_fused_block(v, pc, P, Mc, Kr, nrb, nw, gw, Y, ystride, nent, M32, colsC, C, Crs, W, cap, err);
 }
 
 __global__ __launch_bounds__(256) void crt_recon_carry_kernel(
@@ -234,6 +234,10 @@ struct mgx_crt_plan {
     int8_t *d_cst = nullptr;
     int8_t *d_cstb = nullptr;   // the reduce table in 1 KB blocks (rg_table_blocked), when the tiled reduce takes it
     int8_t *d_mc = nullptr;
+    // the one-pass reconstruct's copy of d_mc with the gamma rows: rows 4 nw + t hold the digits of the
+    // per-prime gamma constants (gw = 2 extra words), when they add no row block; else null and gw = 0
+    int8_t *d_mcg = nullptr;
+    int64_t gw = 0, Lg = 0;
     uint32_t *d_m32 = nullptr;
 };
 
@@ -449,12 +453,38 @@ int plan_create_slices(mgx_crt_plan **out, int64_t K, int64_t bitsA, int64_t bit
     for (size_t i = 0; i < M.size() && i < m32.size(); i++) m32[i] = M[i];
     std::vector<int8_t> cstb;
     if (S == 2 && pl->kb > RES_KB) cstb = rg_table_blocked(cst.data(), P, pl->kb);
+    // Gamma rows for the one-pass reconstruct: with f_q = floor(2^45 / p_q) in balanced digits f_qj
+    // (j = 0..3), row 4 nw + t, column 2q + s holds f_q,(t - s), so the GEMM's words nw and nw + 1 of an
+    // entry carry sum_q y_q f_q (y_q = y_q0 + 256 y_q1, its two digits): the fixed-point gamma the
+    // reconstruct would otherwise sum on the vector ALUs.  Only when the two extra words add no row block.
+    std::vector<int8_t> mcg;
+    if (S == 2 && ceil_div(4 * pl->nw + 8, RECON_ROWS) == ceil_div(4 * pl->nw, RECON_ROWS)) {
+        pl->Lg = round_up(4 * pl->nw + 8, RECON_ROWS);
+        mcg.assign((size_t)pl->Lg * pl->Kr, 0);
+        std::copy(mc.begin(), mc.begin() + (size_t)std::min(pl->Lbp, 4 * pl->nw) * pl->Kr, mcg.begin());
+        bool ok = true;
+        for (int64_t q = 0; q < P && ok; q++) {
+            int32_t vv = pc[q].f45, fd[4];
+            for (int j = 0; j < 4; j++) {
+                fd[j] = ((vv + 128) & 255) - 128;
+                vv = (vv - fd[j]) >> 8;
+            }
+            ok = vv == 0 && pc[q].f45 > 0;
+            for (int t = 0; t < 5 && ok; t++)
+                for (int sd = 0; sd < 2; sd++)
+                    if (t - sd >= 0 && t - sd < 4)
+                        mcg[(size_t)(4 * pl->nw + t) * pl->Kr + 2 * q + sd] = (int8_t)fd[t - sd];
+        }
+        if (ok) pl->gw = 2;
+        else mcg.clear();
+    }
 
     hipError_t e;
     if ((e = hipMalloc(&pl->d_pc, sizeof(PrimeConst) * P)) != hipSuccess ||
         (e = hipMalloc(&pl->d_cst, cst.size())) != hipSuccess || (e = hipMalloc(&pl->d_mc, mc.size())) != hipSuccess ||
         (e = hipMalloc(&pl->d_m32, m32.size() * 4)) != hipSuccess ||
-        (!cstb.empty() && (e = hipMalloc(&pl->d_cstb, cstb.size())) != hipSuccess)) {
+        (!cstb.empty() && (e = hipMalloc(&pl->d_cstb, cstb.size())) != hipSuccess) ||
+        (!mcg.empty() && (e = hipMalloc(&pl->d_mcg, mcg.size())) != hipSuccess)) {
         mgx_crt_plan_destroy(pl);
         return fail(MGX_ERR_OOM, "plan allocation: %s", hipGetErrorString(e));
     }
@@ -463,6 +493,7 @@ int plan_create_slices(mgx_crt_plan **out, int64_t K, int64_t bitsA, int64_t bit
     MGX_HIP(hipMemcpy(pl->d_mc, mc.data(), mc.size(), hipMemcpyHostToDevice));
     MGX_HIP(hipMemcpy(pl->d_m32, m32.data(), m32.size() * 4, hipMemcpyHostToDevice));
     if (!cstb.empty()) MGX_HIP(hipMemcpy(pl->d_cstb, cstb.data(), cstb.size(), hipMemcpyHostToDevice));
+    if (!mcg.empty()) MGX_HIP(hipMemcpy(pl->d_mcg, mcg.data(), mcg.size(), hipMemcpyHostToDevice));
     *out = pl;
     return MGX_OK;
 }
@@ -524,6 +555,7 @@ int mgx_crt_plan_destroy(mgx_crt_plan *pl)
     hipFree(pl->d_pc);
     hipFree(pl->d_cst);
     hipFree(pl->d_cstb);
+    hipFree(pl->d_mcg);
     hipFree(pl->d_mc);
     hipFree(pl->d_m32);
     delete pl;
@@ -651,8 +683,9 @@ int reconstruct_with(const mgx_crt_plan *pl, int64_t K, int64_t N, const uint32_
         if (round_up(nblk, 8) > 0x7fffffffll) return fail(MGX_ERR_SHAPE, "product too large for one reconstruct launch");
         if (nblk > 0)
             hipLaunchKernelGGL(crt_recon_fused_kernel, dim3((unsigned)round_up(nblk, 8)), dim3(256), 0, s, pl->d_pc,
-                               (int)pl->P, pl->d_mc, (int)pl->Kr, nrb, (int)pl->nw, Y, ystride, nent, pl->d_m32, N,
-                               (uint64_t *)C, Crs, cap + 1, cap, device_error_word(), nblk);
+                               (int)pl->P, pl->gw ? pl->d_mcg : pl->d_mc, (int)pl->Kr, nrb, (int)pl->nw, Y, ystride,
+                               nent, pl->d_m32, N, (uint64_t *)C, Crs, cap + 1, cap, device_error_word(), nblk,
+                               (int)pl->gw);
         MGX_HIP(hipGetLastError());
         return MGX_OK;
     }
@@ -888,6 +921,7 @@ __global__ void zero_entries_kernel(uint64_t *C, int64_t Crs, int64_t K, int64_t
 int plan_view(const mgx_crt_plan *pl, PlanView *v)
 {
     v->pc = pl->d_pc; v->cst = pl->d_cst; v->cstb = pl->d_cstb; v->mc = pl->d_mc; v->m32 = pl->d_m32;
+    v->mcg = pl->d_mcg; v->gw = pl->gw;
     v->S = pl->S; v->P = pl->P; v->kb = pl->kb; v->Kr = pl->Kr; v->rows4pad = pl->rows4pad; v->Lbp = pl->Lbp;
     v->out_limbs = pl->out_limbs;
     v->nw = pl->nw;

@@ -1815,7 +1815,7 @@ __device__ __forceinline__ void recon_carry_block(int64_t blk, const uint32_t *_
 __device__ __forceinline__ int recon_block_start(int b, int nw, int nrb) { return b == 0 ? 0 : nw - 32 * (nrb - b); }
 
 __device__ __forceinline__ void recon_fused_block(int64_t eb, const PrimeConst *__restrict__ pc, int P,
-    const int8_t *__restrict__ Mc, int Kr, int nrb, int nw, const uint32_t *__restrict__ Y, int64_t ystride,
+    const int8_t *__restrict__ Mc, int Kr, int nrb, int nw, int gw, const uint32_t *__restrict__ Y, int64_t ystride,
     int64_t nent, const uint32_t *__restrict__ M32, int64_t colsC, uint64_t *__restrict__ C, int64_t Crs, int64_t W,
     int64_t cap, unsigned *__restrict__ err)
 {
@@ -1832,12 +1832,20 @@ __device__ __forceinline__ void recon_fused_block(int64_t eb, const PrimeConst *
     int64_t gsum[2] = {0, 0};
     int64_t wv[32];
 
-    // the top block, with every step's gamma terms
-    const int wst = recon_block_start(T, nw, nrb), nvt = nw - wst;
-    recon_sweep(LS, pc, P, Mc, Kr, 4 * (int64_t)wst, Y, ystride, e0, 2, 0, nrb, acc, gsum);
+    // the top block, with gamma: from its words nw and nw + 1 when Mc carries the gamma rows (gw = 2),
+    // else from every step's gamma terms
+    const int nwg = nw + gw;
+    const int wst = recon_block_start(T, nwg, nrb), nvt = nw - wst;
+    recon_sweep(LS, pc, P, Mc, Kr, 4 * (int64_t)wst, Y, ystride, e0, gw ? 0 : 2, 0, nrb, acc, gsum);
     recon_words(acc, wv);
     int64_t gam;
-    {
+    if (gw) {
+        int64_t F = 0;
+#pragma unroll
+        for (int j = 0; j < 32; j++)
+            F += j == nvt ? wv[j] : j == nvt + 1 ? (int64_t)((uint64_t)wv[j] << 32) : 0;
+        gam = (F + (1ll << 44)) >> 45;
+    } else {
         // lanes l and l + 32 hold the two K halves of the same entries' sums
         int64_t F[2];
 #pragma unroll
@@ -1867,7 +1875,7 @@ __device__ __forceinline__ void recon_fused_block(int64_t eb, const PrimeConst *
             det = H >= (1ll << 28) || H <= -(1ll << 28);
         }
     for (int b = T - 1; b >= 0 && __syncthreads_or(!det); b--) {
-        const int ws = recon_block_start(b, nw, nrb), nv = recon_block_start(b + 1, nw, nrb) - ws;
+        const int ws = recon_block_start(b, nwg, nrb), nv = recon_block_start(b + 1, nwg, nrb) - ws;
         recon_sweep(LS, pc, P, Mc, Kr, 4 * (int64_t)ws, Y, ystride, e0, 0, 0, nrb, acc, gsum);
         recon_words(acc, wv);
 #pragma unroll
@@ -1921,7 +1929,7 @@ __device__ __forceinline__ void recon_fused_block(int64_t eb, const PrimeConst *
         lb[j] = limb;                                                                                   \
     }
     for (int b = 0; b < T; b++) {
-        const int ws = recon_block_start(b, nw, nrb), nv = recon_block_start(b + 1, nw, nrb) - ws;
+        const int ws = recon_block_start(b, nwg, nrb), nv = recon_block_start(b + 1, nwg, nrb) - ws;
         recon_sweep(LS, pc, P, Mc, Kr, 4 * (int64_t)ws, Y, ystride, e0, 0, 0, nrb, acc, gsum);
         recon_words(acc, wv);
         LIMBS(wv, ws, nv);
@@ -1946,8 +1954,9 @@ struct PlanView {
     const PrimeConst *pc;
     const int8_t *cst, *mc;
     const int8_t *cstb;   // the table in 1 KB blocks for the tiled reduce (two-digit plans, kb > RES_KB), else null
+    const int8_t *mcg;    // mc with the gamma rows (gw = 2 words above nw) for the one-pass reconstruct, else null
     const uint32_t *m32;
-    int64_t S, P, kb, Kr, rows4pad, Lbp, out_limbs, nw;
+    int64_t S, P, kb, Kr, rows4pad, Lbp, out_limbs, nw, gw;
     int64_t bitsA, bitsB;   // the operands' bit bounds (|A| < 2^bitsA, |B| < 2^bitsB)
 };
 int plan_view(const mgx_crt_plan *pl, PlanView *v);
