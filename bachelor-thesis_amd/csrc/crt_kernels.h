@@ -299,6 +299,33 @@ __device__ __forceinline__ void reduce_tile(int64_t blk,
 // block over its 128 entries; the residues of a prime block leave through
 // LDS as 8-byte row segments.  KS: 64-byte K steps (kb <= 64 KS; the
 // constant columns kb .. 64 KS are zero).
+// Limbs 4 k32 + 2 hh and 4 k32 + 2 hh + 1 (bytes 32 k32 + 16 hh .. + 15) of
+// the entry at X + base, k32 < K32, for the entry-resident MFMA operands.
+// When every such limb lies below the size word, two limbs per 16-byte load
+// (entries are 8-byte aligned; limbs past the plan's bytes are masked by the
+// caller): half the load instructions of the limb-wise path, whose 64 lanes
+// each touch a cache line of their own.  Else limb-wise, indices clamped at
+// lim - 1 (lim <= W - 1: never the size word's neighbour past the entry).
+template <int K32>
+__device__ __forceinline__ void entry_limbs(const uint64_t *__restrict__ X, int64_t base, int64_t W, int lim, int hh,
+                                            uint64_t (&v)[K32][2])
+{
+    if (4 * K32 <= W - 1) {
+        typedef uint32_t v4u8 __attribute__((ext_vector_type(4), aligned(8)));
+#pragma unroll
+        for (int k32 = 0; k32 < K32; k32++) {
+            const v4u8 t = *(const v4u8 *)(X + base + 4 * k32 + 2 * hh);
+            v[k32][0] = t.x | ((uint64_t)t.y << 32);
+            v[k32][1] = t.z | ((uint64_t)t.w << 32);
+        }
+    } else {
+#pragma unroll
+        for (int k32 = 0; k32 < K32; k32++)
+#pragma unroll
+            for (int u = 0; u < 2; u++) v[k32][u] = X[base + min(4 * k32 + 2 * hh + u, lim - 1)];
+    }
+}
+
 constexpr int RES_KB = 512;   // widest entry (bytes) of the entry-resident tile
 constexpr int RES_KG = 2;   // K steps whose A fragments are read ahead of their MFMAs
 
@@ -337,10 +364,7 @@ __device__ __forceinline__ void reduce_tile_res(int64_t blk,
         const int lim = (int)min<int64_t>(kb / 8, W - 1);
         const int64_t sz = (int64_t)X[base + W - 1];
         uint64_t v[K32][2];
-#pragma unroll
-        for (int k32 = 0; k32 < K32; k32++)
-#pragma unroll
-            for (int u = 0; u < 2; u++) v[k32][u] = X[base + min(4 * k32 + 2 * hh + u, lim - 1)];
+        entry_limbs<K32>(X, base, W, lim, hh, v);
         const int64_t a = real ? min(sz < 0 ? -sz : sz, W - 1) : 0;
         const int nl = (int)min<int64_t>(a, lim);
         neg = real && sz < 0;
@@ -570,30 +594,36 @@ __device__ __forceinline__ void reduce_cres(const PrimeConst *__restrict__ pc, c
         const uint64_t *X = s ? s1.X : s0.X;
         const int lim = (int)min<int64_t>(kb / 8, W - 1);
         rsz = (int64_t)X[base + W - 1];
-#pragma unroll
-        for (int k32 = 0; k32 < K32; k32++)
-#pragma unroll
-            for (int u = 0; u < 2; u++) raw[k32][u] = X[base + min(4 * k32 + 2 * hh + u, lim - 1)];
+        entry_limbs<K32>(X, base, W, lim, hh, raw);
         if (hh == 0 && rreal && over_bound(X + base, min(rsz < 0 ? -rsz : rsz, W - 1), s ? s1.bits : s0.bits))
             atomicOr(err, DEV_ERR_INPUT);
     };
-    int64_t uu = blockIdx.x;
-    if (uu < nu) load(uu);
-    for (; uu < nu; uu += gridDim.x) {
-        // bytes - 128, or 127 - byte for a negative entry (the products then sum
-        // to -|x| plus a constant), masked at |size|
-        const bool neg = rreal && rsz < 0;
-        v4i af[K32];
-        {
-            const int nl = rreal ? (int)min<int64_t>(min(rsz < 0 ? -rsz : rsz, W - 1), (int64_t)(kb / 8)) : 0;
-            const uint64_t flip = neg ? 0x7f7f7f7f7f7f7f7full : 0x8080808080808080ull;
+    // bytes - 128, or 127 - byte for a negative entry (the products then sum
+    // to -|x| plus a constant), masked at |size|
+    auto operand = [&](v4i (&a)[K32], bool &ng) {
+        ng = rreal && rsz < 0;
+        const int nl = rreal ? (int)min<int64_t>(min(rsz < 0 ? -rsz : rsz, W - 1), (int64_t)(kb / 8)) : 0;
+        const uint64_t flip = ng ? 0x7f7f7f7f7f7f7f7full : 0x8080808080808080ull;
 #pragma unroll
-            for (int k32 = 0; k32 < K32; k32++) {
-                const uint64_t d0 = (4 * k32 + 2 * hh < nl ? raw[k32][0] : 0) ^ flip;
-                const uint64_t d1 = (4 * k32 + 2 * hh + 1 < nl ? raw[k32][1] : 0) ^ flip;
-                af[k32] = (v4i){(int)(uint32_t)d0, (int)(uint32_t)(d0 >> 32), (int)(uint32_t)d1, (int)(uint32_t)(d1 >> 32)};
-            }
+        for (int k32 = 0; k32 < K32; k32++) {
+            const uint64_t d0 = (4 * k32 + 2 * hh < nl ? raw[k32][0] : 0) ^ flip;
+            const uint64_t d1 = (4 * k32 + 2 * hh + 1 < nl ? raw[k32][1] : 0) ^ flip;
+            a[k32] = (v4i){(int)(uint32_t)d0, (int)(uint32_t)(d0 >> 32), (int)(uint32_t)d1, (int)(uint32_t)(d1 >> 32)};
         }
+    };
+    int64_t uu = blockIdx.x;
+    v4i af[K32];
+    bool neg = false;
+    if (uu < nu) {
+        load(uu);
+        operand(af, neg);
+    }
+    // the next unit's limbs are loaded before a unit's row blocks and turned
+    // into its operand after the MFMAs of row block cb, before that block's
+    // stores: vmcnt counts stores and loads alike and in order, so waiting
+    // for limbs loaded behind a row block's stores would wait for the stores
+    const int cb = min(1, nrb - 1);
+    for (; uu < nu; uu += gridDim.x) {
         // output register 4g + i of this lane belongs to entry 8g + 4hh + i of
         // the wave; bit 4g + i of m16: that entry is negative
         uint32_t m16;
@@ -605,7 +635,10 @@ __device__ __forceinline__ void reduce_cres(const PrimeConst *__restrict__ pc, c
         const int64_t ul = uu - (s ? nu0 : 0), cbs = kpad >> 6;
         int8_t *const R = s ? s1.R : s0.R;
         const int64_t plane = s ? s1.plane : s0.plane;
-        if (uu + gridDim.x < nu) load(uu + gridDim.x);
+        const bool more = uu + gridDim.x < nu;
+        if (more) load(uu + gridDim.x);
+        v4i afn[K32];
+        bool negn = false;
         // store phase: thread t takes plane t / 8 of the row block and 32
         // entries of the patch: row (t % 8) / 2, columns 32 (t % 2) .. + 31
         const int sp = tid >> 3, sj = tid & 7;
@@ -620,6 +653,7 @@ __device__ __forceinline__ void reduce_cres(const PrimeConst *__restrict__ pc, c
                     const v4i bfr = *(const v4i *)(LC + r * 64 * KS + 16 * ((2 * k32 + hh) ^ cres_swz(r, C)));
                     acc[d] = mfma32(af[k32], bfr, k32 == 0 ? (v16i){0} : acc[d]);
                 }
+            if (more && rb == cb) operand(afn, negn);
             // |acc| <= kb 2^14 <= 2^22, so v = acc_0 + 256 acc_1 is below 2^31 -
             // 2^22 in magnitude and congruent to x - 128 sum 2^(8b) (to -x + 127
             // sum 2^(8b) for a negative entry); u = v + K with K = (128 resp.
@@ -659,6 +693,9 @@ __device__ __forceinline__ void reduce_cres(const PrimeConst *__restrict__ pc, c
             }
             lds_barrier();
         }
+#pragma unroll
+        for (int k32 = 0; k32 < K32; k32++) af[k32] = afn[k32];
+        neg = negn;
     }
 }
 
