@@ -1965,11 +1965,34 @@ __device__ __forceinline__ void recon_fused_block(int64_t eb, const PrimeConst *
         }                                                                                               \
         lb[j] = limb;                                                                                   \
     }
+    // a whole block below the capacity: the same limbs with no per-limb
+    // conditions, the sign applied as (v ^ s) - s, gamma M32 as an unsigned
+    // 32 x 32-bit product corrected for a negative gamma
+    const int64_t smask = neg ? -1 : 0;
+    const uint32_t gamu = (uint32_t)gam, gneg = gam < 0 ? ~0u : 0u;
+    auto limbs_full = [&](int ws) {
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            int64_t x[2];
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                const uint32_t m = M32[ws + 2 * j + u];
+                const uint64_t gm = (uint64_t)gamu * m - ((uint64_t)(gneg & m) << 32);   // gamma M32 mod 2^64
+                const int64_t v = wv[2 * j + u] - (int64_t)gm;
+                x[u] = ((v ^ smask) - smask) + carry;
+                carry = x[u] >> 32;
+            }
+            const uint64_t limb = (uint64_t)(uint32_t)x[0] | ((uint64_t)(uint32_t)x[1] << 32);
+            top = limb ? ws / 2 + j : top;
+            lb[j] = limb;
+        }
+    };
     for (int b = 0; b < T; b++) {
         const int ws = recon_block_start(b, nwg, nrb), nv = recon_block_start(b + 1, nwg, nrb) - ws;
         recon_sweep(LS, pc, P, Mc, Kr, 4 * (int64_t)ws, Y, ystride, e0, 0, 0, nrb, acc, gsum);
         recon_words(acc, wv);
-        LIMBS(wv, ws, nv);
+        if (nv == 32 && ws / 2 + 16 <= lim) limbs_full(ws);
+        else LIMBS(wv, ws, nv);
         emit(ws / 2, nv / 2);
     }
     LIMBS(topw, wst, nvt);
