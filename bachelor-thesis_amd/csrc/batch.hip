@@ -478,7 +478,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
     std::vector<int> pclass(count, -1), crt(count, 0);
     size_t nred[6] = {0, 0, 0, 0, 0, 0}, ngem[4] = {0, 0, 0, 0}, nrec[2] = {0, 0}, nrec_all = 0, nfus = 0;
     std::vector<int> fused(count, 0);
-    const bool fuse_ok = param(MGX_PARAM_RECON) != 1;
+    const int64_t rmode = param(MGX_PARAM_RECON);
     size_t npl[NCLASS] = {0, 0, 0, 0, 0, 0};
     std::vector<AorsJob> aj[WSTAGES];
     std::vector<int64_t> wbytes(count, 0);
@@ -504,7 +504,7 @@ int mgx_batch_create(mgx_batch **out, const mgx_product *prods, int64_t count, v
             nred[2 * reduce_kind(views[i].kb) + sc] += 2;
             ngem[2 * sc + (d.kpad > (sc ? KCHUNK2 : KCHUNK))]++;
             const PlanView &v = views[i];
-            fused[i] = fuse_ok && sc && ceil_div(4 * v.nw, RECON_ROWS) <= 9 && 4 * v.nw <= v.Lbp;
+            fused[i] = rmode != 1 && sc && (rmode == 2 || ceil_div(4 * v.nw, RECON_ROWS) <= 9) && 4 * v.nw <= v.Lbp;
             if (fused[i]) {
                 nfus++;
             } else {

@@ -296,11 +296,11 @@ typedef enum {
                                      256 bytes with the whole constant table in LDS (one workgroup per CU)
                                      whenever it fits, entries over 512 bytes as a tiled MFMA GEMM;
                                      1 = entry-resident tiles of 128 entries, and 64-row tiles past 512 bytes */
-    MGX_PARAM_RECON = 7           /* reconstruct of two-digit plans in single-product calls: 0 (default) = one
-                                     pass writing C's limbs when the product has a block of 256 entries per CU
-                                     and at most 9 row blocks of 128 result bytes (products up to ~4600 bits),
+    MGX_PARAM_RECON = 7           /* reconstruct of two-digit plans: 0 (default) = one pass writing C's limbs
+                                     when the product has at most 9 row blocks of 128 result bytes (products up
+                                     to ~4600 bits) and, in single-product calls, a block of 256 entries per CU,
                                      else two; 1 = always a word-buffer GEMM pass and a carry pass; 2 = always
-                                     one pass */
+                                     one pass (batches read it when they are created) */
 } mgx_param;
 int mgx_set_param(int param, int64_t value);
 int64_t mgx_get_param(int param);
