@@ -156,6 +156,29 @@ def test_batch_winograd_levels(gpu):
     assert routes == [W1, W1, W2, W2, W2, W2, W1, W1, W1, PLAIN, PLAIN, CRT]
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+def test_batch_reconstruct_modes(gpu, mode):
+    """Reconstruct mode 1 gives every two-digit product the word-buffer GEMM and
+    the carry pass, mode 2 the one-pass reconstruct, row blocks past 9
+    included (the default splits at 9); both exact."""
+    mgx = gpu
+    L = mgx.bind_sched()
+    old = mgx.lib().mgx_get_param(7)
+    mgx.call("mgx_set_param", 7, mode)
+    try:
+        specs = [
+            (16, 16, 16, 3000, CRT),     # 6 row blocks
+            (8, 9, 8, 8000, CRT),        # 17 row blocks
+            (4, 5, 6, 16000, CRT),       # 33 row blocks
+            (33, 20, 17, 700, CRT),
+        ]
+        bufs, prods, wants = build(mgx, specs, 1100 + mode)
+        routes = run_and_check(mgx, L, bufs, prods, wants)
+        assert routes == [CRT] * 4
+    finally:
+        mgx.call("mgx_set_param", 7, old)
+
+
 def test_batch_plans_survive_cache_turnover(gpu):
     """A live batch keeps its plans pinned: after more than 256 other shapes
     have passed through the plan cache, replays are still exact."""
