@@ -552,12 +552,12 @@ extern "C" {
 int mgx_crt_plan_destroy(mgx_crt_plan *pl)
 {
     if (!pl) return MGX_OK;
-    hipFree(pl->d_pc);
-    hipFree(pl->d_cst);
-    hipFree(pl->d_cstb);
-    hipFree(pl->d_mcg);
-    hipFree(pl->d_mc);
-    hipFree(pl->d_m32);
+    (void)hipFree(pl->d_pc);
+    (void)hipFree(pl->d_cst);
+    (void)hipFree(pl->d_cstb);
+    (void)hipFree(pl->d_mcg);
+    (void)hipFree(pl->d_mc);
+    (void)hipFree(pl->d_m32);
     delete pl;
     return MGX_OK;
 }

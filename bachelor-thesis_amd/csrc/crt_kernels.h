@@ -1473,16 +1473,17 @@ __device__ __forceinline__ int64_t gamma_term(uint32_t wd, int sub, int q, int P
 // operands of every 64-byte K step brought into a ring of RS_STAGES LDS
 // stages by global_load_lds_dwordx4, two steps ahead, one barrier per step.
 // Stage layout: Mc rows 128 x 64 bytes (chunk c of row r at slot c ^
-// g2_swz(r)), then the step's 16 word planes of Y, 256 entries each, RS_YPITCH
-// bytes apart so that planes 4 apart start 32 banks apart (a lane's four words
-// of one fragment come from planes 8 kk + 4 h + i: the two half-waves' reads
-// fall in different banks).  Per wave and stage: two Mc DMAs (rows 16 (2 wave
-// + u) ..) and four Y DMAs (planes 4 wave + u); the per-prime gamma
-// constants come through the scalar cache, so that the only vector memory
-// operations in the loop are the DMAs the waits count.
+// g2_swz(r)), then the step's 16 word planes of Y, 256 entries each: plane p
+// at rs_yplane(p) = 1024 p + 128 (p / 4), so that the four planes of a lane's
+// fragment (8 kk + 4 h + i, i < 4) lie a multiple of 256 bytes apart (two
+// reads each take one ds_read2st64_b32) and the two half-waves' groups start
+// 32 banks apart.  Per wave and stage: two Mc DMAs (rows 16 (2 wave + u) ..)
+// and four Y DMAs (planes 4 wave + u); the per-prime gamma constants come
+// through the scalar cache, so that the only vector memory operations in the
+// loop are the DMAs the waits count.
 constexpr int RS_STAGES = 3;
-constexpr int RS_YPITCH = 1056;
-constexpr int RS_STAGE = 128 * 64 + 16 * RS_YPITCH;
+__host__ __device__ constexpr int rs_yplane(int p) { return 1024 * p + 128 * (p / 4); }
+constexpr int RS_STAGE = 128 * 64 + rs_yplane(16);
 // One K sweep of a streaming tile: acc (128 result bytes from Mc row mrow0 on
 // x the 256 entries from e0 on) and the gamma terms of the steps gmode picks
 // (0: none, 1: k = rb mod nrb, 2: every step) added to gsum.  LS: the ring.
@@ -1509,7 +1510,7 @@ __device__ __forceinline__ void recon_sweep(int8_t *LS, const PrimeConst *__rest
         _Pragma("unroll") for (int u = 0; u < 4; u++) {                                                \
             const int gp_ = min(16 * (k_) + 4 * wave_u + u, nyp - 1);                                  \
             dma16(ybase + (int64_t)gp_ * ystride * 4, (uint32_t)(16 * lane),                           \
-                  __builtin_amdgcn_readfirstlane(sb_ + 128 * 64 + (4 * wave_u + u) * RS_YPITCH));     \
+                  __builtin_amdgcn_readfirstlane(sb_ + 128 * 64 + rs_yplane(4 * wave_u + u)));        \
         }                                                                                              \
     }
 
@@ -1533,7 +1534,7 @@ __device__ __forceinline__ void recon_sweep(int8_t *LS, const PrimeConst *__rest
             for (int ct = 0; ct < 2; ct++)
 #pragma unroll
                 for (int i = 0; i < 4; i++)
-                    yf[ct][kk][i] = (int)sy[(8 * kk + 4 * h + i) * (RS_YPITCH / 4) + 64 * wave + 32 * ct + c32];
+                    yf[ct][kk][i] = (int)sy[rs_yplane(8 * kk + 4 * h + i) / 4 + 64 * wave + 32 * ct + c32];
 #pragma unroll
             for (int rt = 0; rt < 4; rt++) {
                 const int row = rt * 32 + c32;
