@@ -1939,11 +1939,15 @@ __device__ __forceinline__ void recon_fused_block(int64_t eb, const PrimeConst *
 #pragma unroll
         for (int j = 0; j < 16; j++) LST[tid * CARRY_PITCH + j] = lb[j];
         __syncthreads();
+        // two limbs per thread and 16-byte store (entries are 8-byte aligned)
+        typedef uint64_t v2u64 __attribute__((ext_vector_type(2), aligned(8)));
         const int n = max(0, min(nl, lim - L0));
-        for (int f = tid; f < RECON_ENT * 16; f += RECON_ENT) {
-            const int ent = f >> 4, j = f & 15;
+        for (int f = tid; f < RECON_ENT * 8; f += RECON_ENT) {
+            const int ent = f >> 3, j = 2 * (f & 7);
             const int64_t base = s_base[ent];
-            if (j < n && base >= 0) C[base + L0 + j] = LST[ent * CARRY_PITCH + j];
+            const uint64_t *src = LST + ent * CARRY_PITCH + j;
+            if (j + 1 < n && base >= 0) *(v2u64 *)(C + base + L0 + j) = (v2u64){src[0], src[1]};
+            else if (j < n && base >= 0) C[base + L0 + j] = src[0];
         }
         __syncthreads();   // the staging is read before the next sweep refills the ring
     };
