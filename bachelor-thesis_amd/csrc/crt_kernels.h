@@ -1815,12 +1815,16 @@ __device__ __forceinline__ void recon_carry_block(int64_t blk, const uint32_t *_
             }
         }
         __syncthreads();
+        // two limbs per thread and 16-byte store (entries are 8-byte aligned)
+        typedef uint64_t v2u64 __attribute__((ext_vector_type(2), aligned(8)));
         const int64_t n = min<int64_t>(16, lim - c0);
         if (n > 0)
-            for (int f = tid; f < CARRY_ENT * 16; f += CARRY_ENT) {
-                const int ent = f >> 4, j = f & 15;
+            for (int f = tid; f < CARRY_ENT * 8; f += CARRY_ENT) {
+                const int ent = f >> 3, j = 2 * (f & 7);
                 const int64_t base = s_base[ent];
-                if (j < n && base >= 0) C[base + c0 + j] = LS[ent * CARRY_PITCH + j];
+                const uint64_t *src = LS + ent * CARRY_PITCH + j;
+                if (j + 1 < n && base >= 0) *(v2u64 *)(C + base + c0 + j) = (v2u64){src[0], src[1]};
+                else if (j < n && base >= 0) C[base + c0 + j] = src[0];
             }
         __syncthreads();
     }
