@@ -1067,12 +1067,24 @@ __device__ __forceinline__ void gemm_tile2(int64_t v,
                             (hold[rt][ct][r >> 1] >> (16 * (r & 1))) & 0xffff;
         }
         __syncthreads();
-        // THREADS/128 rows of 128 words per pass, 512 contiguous bytes per row
-        const int64_t j = j0 + (tid & 127);
-        for (int pass = 0; pass < 64; pass++) {
-            const int row = (Sh::THREADS / 128) * pass + (tid >> 7);
+        // THREADS/32 rows of 128 words per pass, four words per thread: 512
+        // contiguous bytes per row, 16-byte stores when the row's words are
+        // 16-byte aligned (colsC a multiple of 4) and inside C
+        const int64_t jq = j0 + 4 * (tid & 31);
+        const bool vec = (colsC & 3) == 0 && jq + 3 < colsC;
+        for (int pass = 0; pass < Sh::M * 32 / Sh::THREADS; pass++) {
+            const int row = (Sh::THREADS / 32) * pass + (tid >> 5);
             const int64_t i = i0 + row;
-            if (i < rowsC && j < colsC) Yw[i * colsC + j] = LW[row * 128 + (tid & 127)];
+            if (i >= rowsC) continue;
+            const uint4 w = *(const uint4 *)(LW + row * 128 + 4 * (tid & 31));
+            if (vec) {
+                *(uint4 *)(Yw + i * colsC + jq) = w;
+            } else {
+                if (jq < colsC) Yw[i * colsC + jq] = w.x;
+                if (jq + 1 < colsC) Yw[i * colsC + jq + 1] = w.y;
+                if (jq + 2 < colsC) Yw[i * colsC + jq + 2] = w.z;
+                if (jq + 3 < colsC) Yw[i * colsC + jq + 3] = w.w;
+            }
         }
     }
 }
