@@ -164,19 +164,37 @@ __device__ __forceinline__ void columns_to_digits(const Acc96 (&acc)[G], uint32_
 struct ColPair {
     Acc96 lo, hi;
 };
+// Every step's product goes into one running sum per lane (no branch on
+// the lane); column l's sum is complete after step l, so lane l keeps a copy
+// of the running sum then (lo), and hi is the rest.  Lanes l >= cnt take no
+// copy: every step belongs to their lo.
+template <int CNT>
+__device__ __forceinline__ void block_steps(uint32_t a, uint32_t &br, uint32_t l, int cnt, Acc96 &acc, Acc96 &snap)
+{
+#pragma unroll
+    for (int i = 0; i < CNT; i++) {
+        if (CNT != 64 && i >= cnt) break;
+        const uint32_t ai = rdlane(a, i);
+        const uint64_t t = (uint64_t)ai * br + acc.lo;   // one v_mad_u64_u32 with the sum as addend
+        acc.hi += (uint32_t)(t < acc.lo);
+        acc.lo = t;
+        if (l == (uint32_t)i) snap = acc;
+        br = rot1(br);
+    }
+}
 __device__ __noinline__ ColPair block_product_fn(uint32_t a, uint32_t b, int cnt)
 {
     const uint32_t l = lane_id();
     uint32_t br = b;
-    ColPair r = {{0, 0}, {0, 0}};
-#pragma unroll 8
-    for (int i = 0; i < cnt; i++) {
-        uint32_t ai = rdlane(a, i);
-        uint64_t p = (uint64_t)ai * br;
-        if (l >= (uint32_t)i) add96(r.lo, p);
-        else add96(r.hi, p);
-        br = rot1(br);
-    }
+    Acc96 acc = {0, 0}, snap = {0, 0};
+    if (cnt == 64) block_steps<64>(a, br, l, cnt, acc, snap);
+    else block_steps<63>(a, br, l, cnt, acc, snap);
+    if (l >= (uint32_t)cnt) snap = acc;
+    ColPair r;
+    r.lo = snap;
+    unsigned long long bw;
+    r.hi.lo = __builtin_subcll(acc.lo, snap.lo, 0ull, &bw);
+    r.hi.hi = acc.hi - snap.hi - (uint32_t)bw;
     return r;
 }
 // The product is symmetric, so the block with fewer valid words is the one
