@@ -182,12 +182,54 @@ __device__ __forceinline__ void block_steps(uint32_t a, uint32_t &br, uint32_t l
         br = rot1(br);
     }
 }
+// lane l <- lane l + j of its row of 16 (j = 1 .. 7; 0 past the row's end)
+__device__ __forceinline__ uint32_t row_shl(uint32_t v, int j)
+{
+    switch (j) {
+    case 1: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x101, 0xF, 0xF, true);
+    case 2: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x102, 0xF, 0xF, true);
+    case 3: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x103, 0xF, 0xF, true);
+    case 4: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x104, 0xF, 0xF, true);
+    case 5: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x105, 0xF, 0xF, true);
+    case 6: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x106, 0xF, 0xF, true);
+    default: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x107, 0xF, 0xF, true);
+    }
+}
+// Whole blocks: the copy is taken once per group of 8 steps, after the
+// group's last step s, by the group's 8 lanes; lane l's copy then also holds
+// the steps l < i <= s, i.e. a_{l+j} b_{64-j} for j = 1 .. 7 - (l mod 8),
+// which are subtracted afterwards (a_{l+j} by a DPP row shift, b_{64-j} by
+// v_readlane).
+__device__ __forceinline__ void block_steps64_grouped(uint32_t a, uint32_t b, uint32_t l, Acc96 &acc, Acc96 &snap)
+{
+    uint32_t br = b;
+#pragma unroll
+    for (int i = 0; i < 64; i++) {
+        const uint32_t ai = rdlane(a, i);
+        const uint64_t t = (uint64_t)ai * br + acc.lo;
+        acc.hi += (uint32_t)(t < acc.lo);
+        acc.lo = t;
+        if ((i & 7) == 7 && (l >> 3) == (uint32_t)(i >> 3)) snap = acc;
+        br = rot1(br);
+    }
+    Acc96 e = {0, 0};
+#pragma unroll
+    for (int j = 1; j < 8; j++) {
+        // row_shl:j: lane l reads lane l + j of its row of 16 (groups of 8 lie inside a row)
+        uint32_t aj = row_shl(a, j);
+        aj = (int)(l & 7) + j <= 7 ? aj : 0u;
+        add96(e, (uint64_t)aj * rdlane(b, 64 - j));
+    }
+    unsigned long long bw;
+    snap.lo = __builtin_subcll(snap.lo, e.lo, 0ull, &bw);
+    snap.hi = snap.hi - e.hi - (uint32_t)bw;
+}
 __device__ __noinline__ ColPair block_product_fn(uint32_t a, uint32_t b, int cnt)
 {
     const uint32_t l = lane_id();
     uint32_t br = b;
     Acc96 acc = {0, 0}, snap = {0, 0};
-    if (cnt == 64) block_steps<64>(a, br, l, cnt, acc, snap);
+    if (cnt == 64) block_steps64_grouped(a, b, l, acc, snap);
     else block_steps<63>(a, br, l, cnt, acc, snap);
     if (l >= (uint32_t)cnt) snap = acc;
     ColPair r;
