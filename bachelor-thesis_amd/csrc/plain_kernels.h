@@ -200,18 +200,32 @@ __device__ __forceinline__ uint32_t row_shl(uint32_t v, int j)
 // the steps l < i <= s, i.e. a_{l+j} b_{64-j} for j = 1 .. 7 - (l mod 8),
 // which are subtracted afterwards (a_{l+j} by a DPP row shift, b_{64-j} by
 // v_readlane).
+// b enters as its two 16-bit halves, each rotated: a product of a 32-bit word
+// and a 16-bit half is below 2^48, so 64 of them sum below 2^54 and every step
+// is a multiply-add into a 64-bit sum with no carry to detect.
+__device__ __forceinline__ Acc96 join16(uint64_t lo16sum, uint64_t hi16sum)
+{
+    Acc96 r;
+    unsigned long long c;
+    r.lo = __builtin_addcll(lo16sum, hi16sum << 16, 0ull, &c);
+    r.hi = (uint32_t)(hi16sum >> 48) + (uint32_t)c;
+    return r;
+}
 __device__ __forceinline__ void block_steps64_grouped(uint32_t a, uint32_t b, uint32_t l, Acc96 &acc, Acc96 &snap)
 {
-    uint32_t br = b;
+    uint32_t bl = b & 0xffffu, bh = b >> 16;
+    uint64_t sl = 0, sh = 0, cl = 0, ch = 0;
 #pragma unroll
     for (int i = 0; i < 64; i++) {
         const uint32_t ai = rdlane(a, i);
-        const uint64_t t = (uint64_t)ai * br + acc.lo;
-        acc.hi += (uint32_t)(t < acc.lo);
-        acc.lo = t;
-        if ((i & 7) == 7 && (l >> 3) == (uint32_t)(i >> 3)) snap = acc;
-        br = rot1(br);
+        sl = (uint64_t)ai * bl + sl;
+        sh = (uint64_t)ai * bh + sh;
+        if ((i & 7) == 7 && (l >> 3) == (uint32_t)(i >> 3)) { cl = sl; ch = sh; }
+        bl = rot1(bl);
+        bh = rot1(bh);
     }
+    acc = join16(sl, sh);
+    snap = join16(cl, ch);
     Acc96 e = {0, 0};
 #pragma unroll
     for (int j = 1; j < 8; j++) {
