@@ -165,23 +165,9 @@ struct ColPair {
     Acc96 lo, hi;
 };
 // Every step's product goes into one running sum per lane (no branch on
-// the lane); column l's sum is complete after step l, so lane l keeps a copy
-// of the running sum then (lo), and hi is the rest.  Lanes l >= cnt take no
+// the lane); column l's sum is complete after step l, so lane l's copy of the
+// running sum then is its lo, and hi is the rest.  Lanes l >= cnt take no
 // copy: every step belongs to their lo.
-template <int CNT>
-__device__ __forceinline__ void block_steps(uint32_t a, uint32_t &br, uint32_t l, int cnt, Acc96 &acc, Acc96 &snap)
-{
-#pragma unroll
-    for (int i = 0; i < CNT; i++) {
-        if (CNT != 64 && i >= cnt) break;
-        const uint32_t ai = rdlane(a, i);
-        const uint64_t t = (uint64_t)ai * br + acc.lo;   // one v_mad_u64_u32 with the sum as addend
-        acc.hi += (uint32_t)(t < acc.lo);
-        acc.lo = t;
-        if (l == (uint32_t)i) snap = acc;
-        br = rot1(br);
-    }
-}
 // lane l <- lane l + j of its row of 16 (j = 1 .. 7; 0 past the row's end)
 __device__ __forceinline__ uint32_t row_shl(uint32_t v, int j)
 {
@@ -238,13 +224,70 @@ __device__ __forceinline__ void block_steps64_grouped(uint32_t a, uint32_t b, ui
     snap.lo = __builtin_subcll(snap.lo, e.lo, 0ull, &bw);
     snap.hi = snap.hi - e.hi - (uint32_t)bw;
 }
+// The same for a block of cnt < 64 words of a: steps stop at cnt, the last
+// group's copy is taken after step cnt - 1, and lanes l >= cnt take the
+// whole sum (every step is below their column).
+__device__ __forceinline__ void block_steps_grouped(uint32_t a, uint32_t b, uint32_t l, int cnt, Acc96 &acc,
+                                                    Acc96 &snap)
+{
+    uint32_t bl = b & 0xffffu, bh = b >> 16;
+    uint64_t sl = 0, sh = 0, cl = 0, ch = 0;
+#pragma unroll
+    for (int g = 0; g < 8; g++) {
+        if (8 * g >= cnt) break;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            if (8 * g + u >= cnt) break;
+            const uint32_t ai = rdlane(a, 8 * g + u);
+            sl = (uint64_t)ai * bl + sl;
+            sh = (uint64_t)ai * bh + sh;
+            bl = rot1(bl);
+            bh = rot1(bh);
+        }
+        if ((l >> 3) == (uint32_t)g) { cl = sl; ch = sh; }
+    }
+    acc = join16(sl, sh);
+    snap = join16(cl, ch);
+    if (l >= (uint32_t)cnt) { snap = acc; return; }
+    // lane l's copy was taken after step min(8 (l / 8) + 7, cnt - 1)
+    const int r = min((int)(l | 7), cnt - 1) - (int)l;
+    Acc96 e = {0, 0};
+#pragma unroll
+    for (int j = 1; j < 8; j++) {
+        uint32_t aj = row_shl(a, j);
+        aj = j <= r ? aj : 0u;
+        add96(e, (uint64_t)aj * rdlane(b, 64 - j));
+    }
+    unsigned long long bw;
+    snap.lo = __builtin_subcll(snap.lo, e.lo, 0ull, &bw);
+    snap.hi = snap.hi - e.hi - (uint32_t)bw;
+}
+// A few words of a (cnt < 16): the copy at every step, no corrections.
+__device__ __forceinline__ void block_steps_short(uint32_t a, uint32_t b, uint32_t l, int cnt, Acc96 &acc,
+                                                  Acc96 &snap)
+{
+    uint32_t bl = b & 0xffffu, bh = b >> 16;
+    uint64_t sl = 0, sh = 0, cl = 0, ch = 0;
+#pragma unroll
+    for (int i = 0; i < 15; i++) {
+        if (i >= cnt) break;
+        const uint32_t ai = rdlane(a, i);
+        sl = (uint64_t)ai * bl + sl;
+        sh = (uint64_t)ai * bh + sh;
+        if (l == (uint32_t)i) { cl = sl; ch = sh; }
+        bl = rot1(bl);
+        bh = rot1(bh);
+    }
+    acc = join16(sl, sh);
+    snap = join16(cl, ch);
+}
 __device__ __noinline__ ColPair block_product_fn(uint32_t a, uint32_t b, int cnt)
 {
     const uint32_t l = lane_id();
-    uint32_t br = b;
     Acc96 acc = {0, 0}, snap = {0, 0};
     if (cnt == 64) block_steps64_grouped(a, b, l, acc, snap);
-    else block_steps<63>(a, br, l, cnt, acc, snap);
+    else if (cnt < 16) block_steps_short(a, b, l, cnt, acc, snap);
+    else block_steps_grouped(a, b, l, cnt, acc, snap);
     if (l >= (uint32_t)cnt) snap = acc;
     ColPair r;
     r.lo = snap;
