@@ -1,4 +1,4 @@
-// peaks.hip -- on-device measurements of the ceilings the kernels are judged
+// peaks.hip -- on-device measurements of the ceilings the kernels are measured
 // against: the INT8 MFMA issue rate (v_mfma_i32_32x32x32_i8, the residue
 // GEMM's instruction), the 32x32->64-bit integer multiply-add rate
 // (v_mad_u64_u32, the plain route's limb product) and the device-to-device
