@@ -42,6 +42,20 @@ def test_fused_matches_two_pass_and_python(gpu, K, M, N, bits):
     assert is_normalized(C0)
 
 
+def test_fused_gamma_words_either_side_of_a_block_boundary(gpu):
+    """The one-pass reconstruct takes gamma from two extra result words when they
+    fit in the last row block (plans whose word count is not a multiple of 32)
+    and sums it on the vector ALUs otherwise: bit lengths around 1000 bits put
+    the word count on both sides of 64."""
+    for bits in range(980, 1044, 4):
+        A, B = rand_ints(4, 9, bits, bits), rand_ints(9, 5, bits, bits + 1)
+        cap = (2 * bits + 12 + 63) // 64 + 1
+        eA, eB = to_entries(A, cap), to_entries(B, cap)
+        C0 = run(gpu, eA, eB, cap, 2)
+        assert to_ints(C0) == matmul(A, B), bits
+        assert is_normalized(C0)
+
+
 @pytest.mark.parametrize("small_bits", [0, 1, 40, 900])
 def test_fused_far_below_bound(gpu, small_bits):
     # a plan for 3000-bit operands, entries of small_bits bits: |C| is far below
