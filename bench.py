@@ -14,7 +14,9 @@ the CPU baseline.  Prints one JSON line.
 Steps cycle over three input seeds (A, B from seeds 2s-1, 2s for s = 1, 2, 3);
 `value` is the mean wall time of the K timed steps (whole job: feature scan,
 dispatch, cached plan, stage kernels, error check; inputs resident in HBM),
-and the per-seed medians are reported beside it.
+and the per-seed medians are reported beside it.  The default 20 untimed
+warm-up products (about 50 ms) cover the device's clock ramp: after the idle
+set-up the first ten or so products run up to 30% slower per kernel.
 
 --batch times the mixed workload instead (one more JSON line): 512
 independent exact products, n uniform in [16, 512], entry bits log-uniform in
@@ -151,7 +153,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=15)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--bits", type=int, default=2048)
     ap.add_argument("--no-plain", action="store_true")
